@@ -1,0 +1,143 @@
+"""Benchmark: DOF-velocity updates per second of the DG-BGK time march.
+
+Workload (default): the plain FP64 cylinder flow -- 20D x 10D box with a
+cylinder of diameter D, ~100k-triangle graded Delaunay mesh, 16 x 32 = 512
+polar ordinates, Mach 0.5, Kn 0.05, diffuse isothermal wall, the
+reference's in-place velocity sweep.  A step is one full ADVNCE loop body
+(wall eta, update of every (node, ordinate), wall pass, moments and
+collision frequency, residual, forces, next time step); f is resident in
+HBM before the timed region starts.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32]
+
+Prints one JSON line.  value = 3*NELEM*VNPNT*K / wall seconds of the K
+timed steps.  roofline: the update kernel (k_update) against HBM, with
+algorithmic bytes = one read + one write of f per DOF-velocity update,
+averaged over the timed launches with HIP events on the library's stream.
+cpu_baseline: the FP64 C oracle (one core) on a bounded sample of the same
+workload (2 steps).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dg-bgk_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _traffic_from_profiles(workload: str, precision: str):
+    """HBM bytes per update launch from the committed rocprofv3 --pmc summary
+    (profiles/update_kernel_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", "update_kernel_traffic.md")
+    if not os.path.exists(path):
+        return None
+    key = f"traffic_bytes[{workload},{precision}]"
+    for line in open(path):
+        if line.strip().startswith(key):
+            try:
+                return float(line.split("=", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--elems", type=int, default=100_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the time march has no multi-GPU path "
+              "in this build (see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from dgbgk import _lib, cases
+    from dgbgk.solver import Solver
+
+    t0 = time.time()
+    case = cases.config2(target_elems=args.elems)
+    prec = _lib.DGBGK_FP64 if args.precision == "fp64" else _lib.DGBGK_FP32
+    s = Solver(case, precision=prec)
+    setup_s = time.time() - t0
+    E, V = case.mesh.nelem, case.nvel
+    dofv = 3 * E * V
+    print(f"# workload cylinder E={E} V={V} ({case.rorder}x{case.torder}) "
+          f"{args.precision} setup {setup_s:.1f}s", file=sys.stderr, flush=True)
+
+    s.step(args.warmup, records=False)
+    s.sync()
+    s.timing(True)
+    t1 = time.perf_counter()
+    recs = s.step(args.steps)
+    s.sync()
+    t2 = time.perf_counter()
+    wall = t2 - t1
+    upd_ms, upd_n = s.kernel_time(0)
+    names = {0: "update", 1: "wall", 2: "moments", 3: "eta", 4: "final", 5: "points"}
+    kern = {names[k]: s.kernel_time(k) for k in names}
+    s.timing(False)
+    print("# kernel ms/launch: " + ", ".join(
+        f"{n}={ms / max(c, 1):.4f}" for n, (ms, c) in kern.items() if c), file=sys.stderr)
+
+    bpv = s.bytes_per_value()
+    value = dofv * args.steps / wall
+    avg_upd_s = (upd_ms / max(upd_n, 1)) / 1e3
+    alg_bytes = 2.0 * bpv * dofv  # one read + one write of f per update
+    achieved = alg_bytes / avg_upd_s / 1e9 if upd_n else None
+    traffic = _traffic_from_profiles(f"cylinder{E}x{V}", args.precision)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle_py import OracleRun
+        o = OracleRun(case)
+        tc0 = time.perf_counter()
+        o.step(args.cpu_steps)
+        tc = time.perf_counter() - tc0
+        cpu = {"value": dofv * args.cpu_steps / tc, "unit": "DOF-velocity updates/s",
+               "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same cylinder workload "
+                         f"(E={E}, V={V}), FP64 C oracle, reference order, {tc:.1f} s"}
+        o.close()
+
+    out = {
+        "metric": "DOF-velocity updates/s + time-to-residual 1e-6, learned vs plain, 1 MI355X",
+        "value": value,
+        "unit": "DOF-velocity updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 (moments f64)",
+        "data": "synthetic (seeded graded Delaunay cylinder mesh, freestream Maxwellian start)",
+        "config": {"workload": "cylinder flow, plain method (BASELINE config 2)",
+                   "nelem": E, "npoin": case.mesh.npoin, "ordinates": V,
+                   "rorder": case.rorder, "torder": case.torder, "mach": 0.5, "kn": 0.05,
+                   "order": "in-place velocity sweep", "precision": args.precision},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": traffic,
+                     "kernel": "k_update", "alg_bytes_per_launch": alg_bytes,
+                     "avg_launch_ms": avg_upd_s * 1e3},
+        "kernels_ms_per_launch": {n: ms / c for n, (ms, c) in kern.items() if c},
+        "residual": {"first": recs[0]["sumres"], "last": recs[-1]["sumres"],
+                     "logres_last": recs[-1]["logres"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

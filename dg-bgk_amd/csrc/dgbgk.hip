@@ -1,0 +1,1317 @@
+// dgbgk.hip -- HIP kernels (gfx950) and the C ABI of the DG-BGK time march.
+//
+// One step (src/advnce.f90:220-449) is a fixed sequence of launches on the
+// context's stream, with no host synchronisation:
+//   k_eta      FLUCON2 wall mass-flux parameter per type-4 side
+//   k_update   all ordinates of all elements: GTEQNF + GETELC + EDGFLXOPT +
+//              GETINC + ADTHEM + INFLOW, element tiles staged in LDS
+//   k_wall     in-place sweep order: reflected wall fluxes, ordinate by
+//              ordinate per (wall element, speed ring)
+//   k_moments  MACROS + COMPMU on f^{k+1}: FP64 compensated sums
+//   k_points   point averages (GETMAC) when the reference gathers them
+//   k_final    residual max, mass, forces, next DTE (ALOTIM2)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dgbgk.h"
+#include "kernels.hpp"
+#include "prep.hpp"
+
+namespace dgbgk {
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+
+// The hardware hands consecutive workgroup ids to the 8 XCDs in turn.  Map
+// them back so each XCD works on one contiguous run of logical blocks: the
+// tiles of a run are mesh neighbours and share halo data in that XCD's L2.
+__device__ __forceinline__ int xcd_logical(int bid, int nblocks) {
+  const int q = nblocks >> 3, r = nblocks & 7;
+  const int xcd = bid & 7, loc = bid >> 3;
+  return xcd * q + min(xcd, r) + loc;
+}
+
+__device__ __forceinline__ float dexp(float x) { return __expf(x); }
+__device__ __forceinline__ double dexp(double x) { return exp(x); }
+
+template <typename T>
+__device__ __forceinline__ void edge_rhsi(T ux, T uy, T nx, T ny, T alen, T g1, T g2,
+                                          T& r1, T& r2) {
+  // normal flux of the upwind values, times the 1-D P1 mass matrix
+  // [[2,1],[1,2]] * length/6 (src/edgflxopt.f90:108-120)
+  const T fn1 = nx * (ux * g1) + ny * (uy * g1);
+  const T fn2 = nx * (ux * g2) + ny * (uy * g2);
+  r1 = -(fn1 * T(2) * alen) - fn2 * alen;
+  r2 = -(fn1 * alen) - fn2 * T(2) * alen;
+}
+
+__device__ __forceinline__ double block_max(double x, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_down(x, o, 64));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 1; i < nw; ++i) x = fmax(x, sh[i]);
+  }
+  return x;  // valid in thread 0
+}
+
+__device__ __forceinline__ double block_sum(double x, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 1; i < nw; ++i) x += sh[i];
+  }
+  return x;
+}
+
+// Neumaier compensated accumulation
+__device__ __forceinline__ void nsum(double& s, double& c, double x) {
+  const double t = s + x;
+  c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+  s = t;
+}
+
+// ---------------------------------------------------------------------------
+// k_update: one block = one tile of <= 256 elements (owned + halo) for a
+// chunk of ordinates.  Per ordinate: phase A computes every slot's predictor
+// (GTEQNF + GETELC's UMEAN, src/getelc.f90:95-138) into LDS; phase B gives
+// each owned element its volume term, its three upwind edge fluxes
+// (element-centric EDGFLXOPT: each element recomputes the flux of its own
+// sides, so there is no scatter and no atomic), GETINC, ADTHEM and INFLOW.
+// ---------------------------------------------------------------------------
+template <typename T, bool kGS>
+__global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
+  __shared__ T s_f[2][kSlots * 3];
+  __shared__ T s_u[2][kSlots];
+  __shared__ double s_red[8];
+  const int logical = xcd_logical(blockIdx.x, gridDim.x);
+  const int tile = logical % a.ntiles;
+  const int v0 = (logical / a.ntiles) * a.vchunk;
+  const int v1 = min(a.V, v0 + a.vchunk);
+  const int e0 = a.tiles[4 * tile], nown = a.tiles[4 * tile + 1];
+  const int hoff = a.tiles[4 * tile + 2], nh = a.tiles[4 * tile + 3];
+  const int slot = threadIdx.x;
+  const bool owned = slot < nown;
+  const bool active = slot < nown + nh;
+  const int e = owned ? e0 + slot : (active ? a.halo[hoff + slot - nown] : 0);
+  const size_t N3 = 3 * (size_t)a.E;
+  const int E = a.E, V = a.V;
+
+  T gx[3] = {}, gy[3] = {}, pA[3] = {}, pB[3] = {}, pU[3] = {}, pV[3] = {}, pM[3] = {};
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      gx[k] = a.geo[k * E + e];
+      gy[k] = a.geo[(3 + k) * E + e];
+      const size_t nd = 3 * (size_t)e + k;
+      pA[k] = a.np[nd];
+      pB[k] = a.np[N3 + nd];
+      pU[k] = a.np[2 * N3 + nd];
+      pV[k] = a.np[3 * N3 + nd];
+      pM[k] = a.np[4 * N3 + nd];
+    }
+  }
+  T area = 0, mmat = 0, enx[3] = {}, eny[3] = {}, eal[3] = {};
+  int info[3] = {1, 1, 1}, nsl[3] = {-1, -1, -1}, mask = 0, wsl = -1;
+  if (owned) {
+    area = a.geo[6 * E + e];
+    mmat = a.geo[7 * E + e];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      info[k] = a.einfo[k * E + e];
+      nsl[k] = a.eslot[k * E + e];
+      enx[k] = a.enx[k * E + e];
+      eny[k] = a.eny[k * E + e];
+      eal[k] = a.ealen[k * E + e];
+    }
+    mask = a.emask[e];
+    if (kGS) wsl = a.ewslot[e];
+  }
+  const T dte = (T)a.scal[0];
+  const T c13 = T(1) / T(3);
+  const T alpha = a.alpha;
+  double res = 0.0;
+
+  for (int v = v0; v < v1; ++v) {
+    const int buf = (v - v0) & 1;
+    const T ux = a.vtab[v], uy = a.vtab[V + v];
+    T f[3] = {}, coll[3] = {}, U = 0;
+    if (active) {
+      const T* fp = a.fcur + (size_t)v * N3 + 3 * (size_t)e;
+      f[0] = fp[0];
+      f[1] = fp[1];
+      f[2] = fp[2];
+      T dive = 0, cs = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const T dx = pU[k] - ux, dy = pV[k] - uy;
+        const T feq = pA[k] * dexp(-pB[k] * (dx * dx + dy * dy));
+        coll[k] = pM[k] * (feq - f[k]);
+        dive += gx[k] * (ux * f[k]) + gy[k] * (uy * f[k]);
+        cs += coll[k];
+      }
+      dive = dive - c13 * cs;
+      U = (dive * dte) * T(0.5);
+      s_f[buf][3 * slot + 0] = f[0];
+      s_f[buf][3 * slot + 1] = f[1];
+      s_f[buf][3 * slot + 2] = f[2];
+      s_u[buf][slot] = U;
+    }
+    __syncthreads();
+    if (!owned) continue;
+    // volume term (src/getelc.f90:142-172)
+    const T fxme = c13 * (ux * (f[0] - U) + ux * (f[1] - U) + ux * (f[2] - U));
+    const T fyme = c13 * (uy * (f[0] - U) + uy * (f[1] - U) + uy * (f[2] - U));
+    T rhs[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rhs[k] = (gx[k] * area) * fxme + (gy[k] * area) * fyme;
+    // edges (src/edgflxopt.f90:72-125, src/getbou.f90:142-281)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int in = info[k];
+      const int kind = in & 3;
+      if (kind == 1) continue;  // inflow side: no flux
+      const int a1 = (in >> 2) & 3, a2 = (in >> 4) & 3;
+      T g1, g2, r1, r2;
+      if (kind == 0) {
+        const bool left = (in >> 10) & 1;
+        const T scpr = ux * enx[k] + uy * eny[k];
+        if ((scpr > T(0)) == left) {
+          g1 = f[a1] - U;
+          g2 = f[a2] - U;
+        } else {
+          const int ns = nsl[k], b1 = (in >> 6) & 3, b2 = (in >> 8) & 3;
+          const T un = s_u[buf][ns];
+          g1 = s_f[buf][3 * ns + b1] - un;
+          g2 = s_f[buf][3 * ns + b2] - un;
+        }
+        edge_rhsi(ux, uy, enx[k], eny[k], eal[k], g1, g2, r1, r2);
+        if (left) { rhs[a1] += r1; rhs[a2] += r2; }
+        else { rhs[a1] -= r1; rhs[a2] -= r2; }
+        continue;
+      }
+      int i1 = -1, ws = -1;
+      if (kind == 3) {
+        ws = nsl[k];
+        i1 = a.refl_i[2 * ((size_t)ws * V + v)];
+      }
+      if (i1 < 0) {  // outflow, or molecules moving into the wall
+        g1 = f[a1] - U;
+        g2 = f[a2] - U;
+      } else if (!kGS) {  // reflected distribution, previous-step values
+        const int i2 = a.refl_i[2 * ((size_t)ws * V + v) + 1];
+        const T* rt = a.refl_t + 3 * ((size_t)ws * V + v);
+        const T zd1 = rt[0], zd2 = rt[1], mw = rt[2];
+        const T* fb = a.fcur + 3 * (size_t)e;
+        const T fa1 = fb[(size_t)i1 * N3 + a1], fb1 = fb[(size_t)i2 * N3 + a1];
+        const T fa2 = fb[(size_t)i1 * N3 + a2], fb2 = fb[(size_t)i2 * N3 + a2];
+        const T rf1 = (zd2 * fa1 + zd1 * fb1) / (zd1 + zd2);
+        const T rf2 = (zd2 * fa2 + zd1 * fb2) / (zd1 + zd2);
+        const T mf = mw * (T)a.eta[a.wall_bside[ws]];
+        g1 = (T(1) - alpha) * rf1 + alpha * mf;
+        g2 = (T(1) - alpha) * rf2 + alpha * mf;
+      } else {
+        continue;  // the ordered wall pass adds it
+      }
+      edge_rhsi(ux, uy, enx[k], eny[k], eal[k], g1, g2, r1, r2);
+      rhs[a1] += r1;
+      rhs[a2] += r2;
+    }
+    if (kGS && wsl >= 0) {
+      T* wp = a.wpart + ((size_t)wsl * V + v) * 6;
+      wp[0] = rhs[0]; wp[1] = rhs[1]; wp[2] = rhs[2];
+      wp[3] = coll[0]; wp[4] = coll[1]; wp[5] = coll[2];
+      continue;
+    }
+    // GETINC + ADTHEM + INFLOW (src/getinc.f90:30-42, src/adthem.f90:109-118,
+    // src/inflow.f90:165-181)
+    T* fo = a.fnxt + (size_t)v * N3 + 3 * (size_t)e;
+    const T fin = a.vtab[2 * V + v];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const T d = mmat * (dte * rhs[k]) + coll[k] * dte;
+      res = fmax(res, (double)fabs(d));
+      T fn = f[k] + d;
+      fn = fn < T(0) ? T(0) : fn;
+      if ((mask >> k) & 1) fn = fin;
+      fo[k] = fn;
+    }
+  }
+  const double r = block_max(res, s_red);
+  if (threadIdx.x == 0) a.res_part[logical] = r;
+}
+
+// ---------------------------------------------------------------------------
+// k_wall: the reference's in-place velocity sweep at wall elements.  The
+// reflected flux of ordinate v reads f at ordinates I1, I2 of the same
+// element; those already advanced this step (index < v) are read from the
+// new buffer (src/getbou.f90:245-250 inside src/advnce.f90:273-383).  One
+// thread per (wall element, speed ring) when every reflection stays in its
+// ring, else per wall element over all ordinates.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
+  __shared__ double s_red[8];
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  double res = 0.0;
+  if (gid < a.nwe * a.ngroups) {
+    const int we = gid / a.ngroups, grp = gid % a.ngroups;
+    const int e = a.wall_elems[we];
+    const int E = a.E, V = a.V;
+    const size_t N3 = 3 * (size_t)E;
+    const int vb = a.ngroups > 1 ? grp * a.T_ : 0;
+    const int ve = a.ngroups > 1 ? vb + a.T_ : V;
+    const T mmat = a.geo[7 * E + e];
+    int info[3], nsl[3];
+    T enx[3], eny[3], eal[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      info[k] = a.einfo[k * E + e];
+      nsl[k] = a.eslot[k * E + e];
+      enx[k] = a.enx[k * E + e];
+      eny[k] = a.eny[k * E + e];
+      eal[k] = a.ealen[k * E + e];
+    }
+    const int mask = a.emask[e];
+    const T dte = (T)a.scal[0];
+    const T alpha = a.alpha;
+    for (int v = vb; v < ve; ++v) {
+      const T* wp = a.wpart + ((size_t)we * V + v) * 6;
+      T rhs[3] = {wp[0], wp[1], wp[2]};
+      const T coll[3] = {wp[3], wp[4], wp[5]};
+      const T ux = a.vtab[v], uy = a.vtab[V + v];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if ((info[k] & 3) != 3) continue;
+        const int ws = nsl[k];
+        const size_t rk = (size_t)ws * V + v;
+        const int i1 = a.refl_i[2 * rk];
+        if (i1 < 0) continue;
+        const int i2 = a.refl_i[2 * rk + 1];
+        const int a1 = (info[k] >> 2) & 3, a2 = (info[k] >> 4) & 3;
+        const T zd1 = a.refl_t[3 * rk], zd2 = a.refl_t[3 * rk + 1], mw = a.refl_t[3 * rk + 2];
+        const T* s1 = (i1 < v ? a.fnxt : a.fcur) + (size_t)i1 * N3 + 3 * (size_t)e;
+        const T* s2 = (i2 < v ? a.fnxt : a.fcur) + (size_t)i2 * N3 + 3 * (size_t)e;
+        const T rf1 = (zd2 * s1[a1] + zd1 * s2[a1]) / (zd1 + zd2);
+        const T rf2 = (zd2 * s1[a2] + zd1 * s2[a2]) / (zd1 + zd2);
+        const T mf = mw * (T)a.eta[a.wall_bside[ws]];
+        const T g1 = (T(1) - alpha) * rf1 + alpha * mf;
+        const T g2 = (T(1) - alpha) * rf2 + alpha * mf;
+        T r1, r2;
+        edge_rhsi(ux, uy, enx[k], eny[k], eal[k], g1, g2, r1, r2);
+        rhs[a1] += r1;
+        rhs[a2] += r2;
+      }
+      const T* fc = a.fcur + (size_t)v * N3 + 3 * (size_t)e;
+      T* fo = a.fnxt + (size_t)v * N3 + 3 * (size_t)e;
+      const T fin = a.vtab[2 * V + v];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const T d = mmat * (dte * rhs[k]) + coll[k] * dte;
+        res = fmax(res, (double)fabs(d));
+        T fn = fc[k] + d;
+        fn = fn < T(0) ? T(0) : fn;
+        if ((mask >> k) & 1) fn = fin;
+        fo[k] = fn;
+      }
+    }
+  }
+  const double r = block_max(res, s_red);
+  if (threadIdx.x == 0) a.res_part[blockIdx.x] = r;
+}
+
+// ---------------------------------------------------------------------------
+// k_moments: MACROS (src/macros.f90:80-164) and COMPMU (src/compmu.f90:
+// 299-347) per DG node, FP64 compensated sums over all ordinates.  Pass 1:
+// n, n*u, n*v; pass 2: pressure and mean relative speed about u.  Writes the
+// FP64 node fields and the next step's GTEQNF/COMPMU parameters in T.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
+  __shared__ double s_red[8];
+  const size_t N3 = 3 * (size_t)a.E;
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int V = a.V;
+  double mu = 0.0, mass = 0.0;
+  if (k < N3) {
+    const T* fp = a.f + k;
+    const double* cxa = a.vtd;
+    const double* cya = a.vtd + V;
+    const double* cxb = a.vtd + 2 * V;
+    const double* cyb = a.vtd + 3 * V;
+    const double* wv = a.vtd + 4 * V;
+    double s0 = 0, c0 = 0, s1 = 0, c1 = 0, s2 = 0, c2 = 0;
+    int v = 0;
+    for (; v + 4 <= V; v += 4) {
+      double fv[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) fv[j] = (double)fp[(size_t)(v + j) * N3];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double fw = fv[j] * wv[v + j];
+        nsum(s0, c0, fw);
+        nsum(s1, c1, cxb[v + j] * fw);
+        nsum(s2, c2, cyb[v + j] * fw);
+      }
+    }
+    for (; v < V; ++v) {
+      const double fw = (double)fp[(size_t)v * N3] * wv[v];
+      nsum(s0, c0, fw);
+      nsum(s1, c1, cxb[v] * fw);
+      nsum(s2, c2, cyb[v] * fw);
+    }
+    const double n = s0 + c0;
+    double nd = 0.0, ux = 0.0, uy = 0.0;
+    if (n >= 1e-20) {
+      nd = n;
+      ux = (s1 + c1) / n;
+      uy = (s2 + c2) / n;
+    }
+    double sp = 0, cp = 0, st = 0, ct = 0;
+    const bool dens = n >= 1e-20;
+    v = 0;
+    for (; v + 4 <= V; v += 4) {
+      double fv[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) fv[j] = (double)fp[(size_t)(v + j) * N3];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double fw = fv[j] * wv[v + j];
+        const double dx = cxb[v + j] - ux, dy = cyb[v + j] - uy;
+        nsum(sp, cp, 0.5 * (dx * dx + dy * dy) * fw);
+        if (dens) {
+          const double ex = ux - cxa[v + j], ey = uy - cya[v + j];
+          nsum(st, ct, sqrt(ex * ex + ey * ey) * fw);
+        }
+      }
+    }
+    for (; v < V; ++v) {
+      const double fw = (double)fp[(size_t)v * N3] * wv[v];
+      const double dx = cxb[v] - ux, dy = cyb[v] - uy;
+      nsum(sp, cp, 0.5 * (dx * dx + dy * dy) * fw);
+      if (dens) {
+        const double ex = ux - cxa[v], ey = uy - cya[v];
+        nsum(st, ct, sqrt(ex * ex + ey * ey) * fw);
+      }
+    }
+    const double ps = (sp + cp) * a.mm;
+    const double ctv = dens ? (st + ct) / n : 0.0;
+    mu = sqrt(2.0) * ctv * a.sigt * nd;
+    a.mom[k] = nd;
+    a.mom[N3 + k] = ux;
+    a.mom[2 * N3 + k] = uy;
+    a.mom[3 * N3 + k] = ps;
+    a.mom[4 * N3 + k] = mu;
+    // GTEQNF parameters (src/gteqnf.f90:420-429)
+    double pa = 0.0, pb = 0.0;
+    if (!(ps < 1e-20)) {
+      pb = (a.mm * nd) / (2 * ps);
+      pa = nd * (pb / 3.1416);
+    }
+    a.np[k] = (T)pa;
+    a.np[N3 + k] = (T)pb;
+    a.np[2 * N3 + k] = (T)ux;
+    a.np[3 * N3 + k] = (T)uy;
+    a.np[4 * N3 + k] = (T)mu;
+    mass = nd * a.rj[k / 3] / 6.0;
+  }
+  const double m1 = block_max(mu, s_red);
+  if (threadIdx.x == 0) a.mu_part[blockIdx.x] = m1;
+  const double m2 = block_sum(mass, s_red);
+  if (threadIdx.x == 0) a.mass_part[blockIdx.x] = m2;
+}
+
+// FLUCON2 (src/flucon2.f90:133-233): one block per type-4 wall side.
+template <typename T>
+__global__ void __launch_bounds__(256) k_eta(const T* __restrict__ f,
+                                             const int* __restrict__ side,
+                                             const double* __restrict__ geo,
+                                             const double* __restrict__ vtd,
+                                             double* __restrict__ eta, int E, int V) {
+  __shared__ double s_red[8];
+  const int j = blockIdx.x;
+  const int b = side[4 * j], e = side[4 * j + 1], in1 = side[4 * j + 2], in2 = side[4 * j + 3];
+  const double anx = geo[3 * j], any = geo[3 * j + 1], bottom = geo[3 * j + 2];
+  const size_t N3 = 3 * (size_t)E;
+  double s = 0.0;
+  for (int v = threadIdx.x; v < V; v += blockDim.x) {
+    const double test = vtd[2 * V + v] * anx + vtd[3 * V + v] * any;
+    if (test > 0.0) {
+      const T* fp = f + (size_t)v * N3 + 3 * (size_t)e;
+      const double nf = 0.5 * ((double)fp[in1] + (double)fp[in2]);
+      s += fabs(test) * nf * vtd[4 * V + v];
+    }
+  }
+  s = block_sum(s, s_red);
+  if (threadIdx.x == 0) eta[b] = s / bottom;
+}
+
+// MACROS point averages (src/macros.f90:156-188) over the point CSR.
+__global__ void __launch_bounds__(256) k_points(const double* __restrict__ mom,
+                                                const double* __restrict__ rj,
+                                                const int* __restrict__ ptr,
+                                                const int* __restrict__ node,
+                                                double* __restrict__ pts,
+                                                double* __restrict__ ps_glob, int E,
+                                                int P, double mm, double rgas,
+                                                int gather) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= P) return;
+  const size_t N3 = 3 * (size_t)E;
+  double flag = 0, nd = 0, u = 0, vv = 0, ps = 0;
+  for (int i = ptr[q]; i < ptr[q + 1]; ++i) {
+    const int k = node[i];
+    const double ae = rj[k / 3];
+    const double n = mom[k];
+    if (n >= 1e-20) {
+      u += ae * mom[N3 + k];
+      vv += ae * mom[2 * N3 + k];
+    }
+    flag += ae;
+    nd += n * ae;
+    ps += mom[3 * N3 + k] * ae;
+  }
+  double rho = 0, t = 0;
+  if (nd < 1e-20) {
+    u = vv = ps = 0.0;
+  } else {
+    nd = nd / flag;
+    rho = nd * mm;
+    u = u / flag;
+    vv = vv / flag;
+    ps = ps / flag;
+    t = ps / (rgas * rho);
+  }
+  pts[q] = nd;
+  pts[P + q] = rho;
+  pts[2 * P + q] = u;
+  pts[3 * P + q] = vv;
+  pts[4 * P + q] = ps;
+  pts[5 * P + q] = t;
+  if (gather) ps_glob[q] = ps;
+}
+
+// Step epilogue: GETRES max, mass, GETFOR (src/getfor.f90:29-49) on the
+// gathered point pressure, the residual record, and ALOTIM2's next DTE.
+__global__ void __launch_bounds__(256) k_final(const double* __restrict__ res_part, int nres,
+                                               const double* __restrict__ mu_part,
+                                               const double* __restrict__ mass_part, int nmom,
+                                               const int* __restrict__ fside, int nf,
+                                               const double* __restrict__ coord,
+                                               const double* __restrict__ ps_glob,
+                                               double* __restrict__ scal,
+                                               StepRec* __restrict__ rec, int itime,
+                                               int record) {
+  __shared__ double s_red[8];
+  double r = 0.0, m = 0.0, ms = 0.0, lift = 0.0, drag = 0.0;
+  for (int i = threadIdx.x; i < nres; i += blockDim.x) r = fmax(r, res_part[i]);
+  for (int i = threadIdx.x; i < nmom; i += blockDim.x) {
+    m = fmax(m, mu_part[i]);
+    ms += mass_part[i];
+  }
+  for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+    const int p1 = fside[2 * i], p2 = fside[2 * i + 1];
+    const double x1 = coord[2 * p1], y1 = coord[2 * p1 + 1];
+    const double x2 = coord[2 * p2], y2 = coord[2 * p2 + 1];
+    const double len = sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
+    const double px = -(y2 - y1), py = x2 - x1;
+    const double mag = sqrt(px * px + py * py);
+    const double psav = 0.5 * (ps_glob[p1] + ps_glob[p2]);
+    lift -= psav * len * (py / mag);
+    drag -= psav * len * (px / mag);
+  }
+  r = block_max(r, s_red);
+  m = block_max(m, s_red);
+  ms = block_sum(ms, s_red);
+  lift = block_sum(lift, s_red);
+  drag = block_sum(drag, s_red);
+  if (threadIdx.x == 0) {
+    if (record >= 0) {
+      if (itime == 1) scal[1] = r;
+      rec[record].sumres = r;
+      rec[record].logres = log(r / scal[1]);
+      rec[record].dte = scal[0];
+      rec[record].mass = ms;
+      rec[record].lift = lift;
+      rec[record].drag = drag;
+    }
+    scal[2] = lift;
+    scal[3] = drag;
+    scal[5] = ms;
+    const double dteb = (m > 0.0) ? 0.9 * (1 / m) : 1000000.0;
+    scal[0] = dteb < scal[4] ? dteb : scal[4];
+  }
+}
+
+// Maxwellian seed per DG node from per-point (n, beta^2, u, v): the INICON2
+// formula (src/inicon2.f90:149-163) with point-wise state.
+template <typename T>
+__global__ void __launch_bounds__(256) k_seed(T* __restrict__ f, const int* __restrict__ intma,
+                                              const double* __restrict__ ps4,
+                                              const double* __restrict__ vtd, int E, int V,
+                                              int P) {
+  const size_t N3 = 3 * (size_t)E;
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (k >= N3) return;
+  const int q = intma[k] - 1;
+  const double n1 = ps4[q], b2 = ps4[P + q], u0 = ps4[2 * P + q], v0 = ps4[3 * P + q];
+  const double ux = vtd[2 * V + v], uy = vtd[3 * V + v];
+  const double tmp = (ux - u0) * (ux - u0) + (uy - v0) * (uy - v0);
+  const double sp = sqrt(tmp);
+  const double f0 = (b2 / 3.1416) * exp(-(b2 * (sp * sp)));
+  f[(size_t)v * N3 + k] = (T)(n1 * f0);
+}
+
+// Layout conversions between the reference's DISNF(3,V,E) (double) and
+// the device's velocity-major T[V][3E].
+template <typename T>
+__global__ void k_to_device(const double* __restrict__ in, T* __restrict__ out, int E, int V) {
+  const size_t N3 = 3 * (size_t)E;
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (k >= N3) return;
+  const size_t e = k / 3, a = k % 3;
+  out[(size_t)v * N3 + k] = (T)in[(e * V + v) * 3 + a];
+}
+
+template <typename T>
+__global__ void k_to_host(const T* __restrict__ in, double* __restrict__ out, int E, int V) {
+  const size_t N3 = 3 * (size_t)E;
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (k >= N3) return;
+  const size_t e = k / 3, a = k % 3;
+  out[(e * V + v) * 3 + a] = (double)in[(size_t)v * N3 + k];
+}
+
+// ---------------------------------------------------------------------------
+// host context
+// ---------------------------------------------------------------------------
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+
+#define HIPCK(x)                                                              \
+  do {                                                                        \
+    hipError_t e_ = (x);                                                      \
+    if (e_ != hipSuccess)                                                     \
+      return fail(kErrHip, std::string(#x) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+template <typename X>
+int upload(X** d, const std::vector<X>& h, std::vector<void*>& owned) {
+  size_t n = std::max<size_t>(h.size(), 1);
+  HIPCK(hipMalloc((void**)d, n * sizeof(X)));
+  owned.push_back(*d);
+  if (!h.empty()) HIPCK(hipMemcpy(*d, h.data(), h.size() * sizeof(X), hipMemcpyHostToDevice));
+  return kOk;
+}
+
+template <typename X>
+int alloc(X** d, size_t n, std::vector<void*>& owned) {
+  HIPCK(hipMalloc((void**)d, std::max<size_t>(n, 1) * sizeof(X)));
+  owned.push_back(*d);
+  return kOk;
+}
+
+}  // namespace
+
+struct Ctx {
+  Prep p;
+  int prec = 0;   // 0 fp64, 1 fp32
+  int gs = 1;     // in-place sweep order
+  size_t tsz = 8;
+  hipStream_t stream = nullptr;
+  std::vector<void*> owned;
+  void* f[2] = {nullptr, nullptr};
+  int cur = 0;
+  void *geo = nullptr, *enx = nullptr, *eny = nullptr, *ealen = nullptr, *np = nullptr,
+       *vtab = nullptr, *refl_t = nullptr, *wpart = nullptr;
+  int *einfo = nullptr, *eslot = nullptr, *emask = nullptr, *ewslot = nullptr,
+      *refl_i = nullptr, *wall_bside = nullptr, *tiles = nullptr, *halo = nullptr,
+      *wall_elems = nullptr, *eta_side = nullptr, *fside = nullptr, *pt_ptr = nullptr,
+      *pt_node = nullptr, *intma = nullptr;
+  double *vtd = nullptr, *eta_geo = nullptr, *eta = nullptr, *rj = nullptr, *mom = nullptr,
+         *pts = nullptr, *ps_glob = nullptr, *coord = nullptr, *res_part = nullptr,
+         *mu_part = nullptr, *mass_part = nullptr, *scal = nullptr, *ps4 = nullptr;
+  StepRec* rec = nullptr;
+  int rec_cap = 0;
+  int ntiles = 0, vchunk = 1, nvchunks = 1, nblk_upd = 0, nblk_wall = 0, ngroups = 1,
+      nblk_mom = 0, neta = 0, nforce = 0;
+  int itime = 0;
+  // timing: events recorded around launches, read after a synchronisation
+  bool timing = false;
+  struct Ev {
+    int kernel;
+    hipEvent_t a, b;
+  };
+  std::vector<Ev> ev_pool;   // all created events
+  size_t ev_used = 0;        // recorded, not yet read
+  double t_ms[8] = {};
+  long t_cnt[8] = {};
+};
+
+namespace {
+
+template <typename T>
+int launch_moments(Ctx& c) {
+  MomArgs<T> m;
+  m.f = (const T*)c.f[c.cur];
+  m.vtd = c.vtd;
+  m.rj = c.rj;
+  m.mom = c.mom;
+  m.np = (T*)c.np;
+  m.mu_part = c.mu_part;
+  m.mass_part = c.mass_part;
+  m.mm = c.p.prm.m / kNa;
+  m.sigt = kPi * c.p.prm.d * c.p.prm.d;
+  m.E = c.p.E;
+  m.V = c.p.V;
+  hipLaunchKernelGGL(k_moments<T>, dim3(c.nblk_mom), dim3(256), 0, c.stream, m);
+  HIPCK(hipGetLastError());
+  return kOk;
+}
+
+int launch_final(Ctx& c, int record) {
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.res_part,
+                     c.nblk_upd + c.nblk_wall, c.mu_part, c.mass_part, c.nblk_mom, c.fside,
+                     c.nforce, c.coord, c.ps_glob, c.scal, c.rec, c.itime, record);
+  HIPCK(hipGetLastError());
+  return kOk;
+}
+
+int launch_points(Ctx& c, int gather) {
+  const int P = c.p.P;
+  hipLaunchKernelGGL(k_points, dim3((P + 255) / 256), dim3(256), 0, c.stream, c.mom, c.rj,
+                     c.pt_ptr, c.pt_node, c.pts, c.ps_glob, c.p.E, P, c.p.prm.m / kNa,
+                     c.p.prm.rgas, gather);
+  HIPCK(hipGetLastError());
+  return kOk;
+}
+
+// Reads every recorded event pair (the stream must be synchronised).
+void tharvest(Ctx& c) {
+  for (size_t i = 0; i < c.ev_used; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c.ev_pool[i].a, c.ev_pool[i].b) == hipSuccess) {
+      c.t_ms[c.ev_pool[i].kernel] += ms;
+      c.t_cnt[c.ev_pool[i].kernel] += 1;
+    }
+  }
+  c.ev_used = 0;
+}
+void tbegin(Ctx& c, int k) {
+  if (!c.timing) return;
+  if (c.ev_used == c.ev_pool.size()) {
+    Ctx::Ev e{k, nullptr, nullptr};
+    if (hipEventCreate(&e.a) != hipSuccess || hipEventCreate(&e.b) != hipSuccess) {
+      c.timing = false;
+      return;
+    }
+    c.ev_pool.push_back(e);
+  }
+  c.ev_pool[c.ev_used].kernel = k;
+  hipEventRecord(c.ev_pool[c.ev_used].a, c.stream);
+}
+void tend(Ctx& c, int k) {
+  if (!c.timing) return;
+  (void)k;
+  hipEventRecord(c.ev_pool[c.ev_used].b, c.stream);
+  c.ev_used += 1;
+}
+
+template <typename T>
+int step_once(Ctx& c, int record) {
+  const Prep& p = c.p;
+  c.itime += 1;
+  const T* fc = (const T*)c.f[c.cur];
+  T* fn = (T*)c.f[1 - c.cur];
+  if (c.neta) {
+    tbegin(c, 3);
+    hipLaunchKernelGGL(k_eta<T>, dim3(c.neta), dim3(256), 0, c.stream, fc, c.eta_side,
+                       c.eta_geo, c.vtd, c.eta, p.E, p.V);
+    HIPCK(hipGetLastError());
+    tend(c, 3);
+  }
+  UpdArgs<T> u;
+  u.fcur = fc; u.fnxt = fn; u.geo = (const T*)c.geo; u.einfo = c.einfo; u.eslot = c.eslot;
+  u.enx = (const T*)c.enx; u.eny = (const T*)c.eny; u.ealen = (const T*)c.ealen;
+  u.emask = c.emask; u.ewslot = c.ewslot; u.np = (const T*)c.np; u.vtab = (const T*)c.vtab;
+  u.refl_i = c.refl_i; u.refl_t = (const T*)c.refl_t; u.wall_bside = c.wall_bside;
+  u.eta = c.eta; u.tiles = c.tiles; u.halo = c.halo; u.wpart = (T*)c.wpart;
+  u.scal = c.scal; u.res_part = c.res_part; u.alpha = (T)p.prm.alpha;
+  u.E = p.E; u.V = p.V; u.ntiles = c.ntiles; u.vchunk = c.vchunk;
+  tbegin(c, 0);
+  if (c.gs && !p.wall_elems.empty())
+    hipLaunchKernelGGL((k_update<T, true>), dim3(c.nblk_upd), dim3(kSlots), 0, c.stream, u);
+  else
+    hipLaunchKernelGGL((k_update<T, false>), dim3(c.nblk_upd), dim3(kSlots), 0, c.stream, u);
+  HIPCK(hipGetLastError());
+  tend(c, 0);
+  if (c.nblk_wall) {
+    WallArgs<T> w;
+    w.fcur = fc; w.fnxt = fn; w.geo = (const T*)c.geo; w.einfo = c.einfo; w.eslot = c.eslot;
+    w.enx = (const T*)c.enx; w.eny = (const T*)c.eny; w.ealen = (const T*)c.ealen;
+    w.emask = c.emask; w.vtab = (const T*)c.vtab; w.refl_i = c.refl_i;
+    w.refl_t = (const T*)c.refl_t; w.wall_bside = c.wall_bside; w.eta = c.eta;
+    w.wall_elems = c.wall_elems; w.wpart = (const T*)c.wpart; w.scal = c.scal;
+    w.res_part = c.res_part + c.nblk_upd; w.alpha = (T)p.prm.alpha; w.E = p.E; w.V = p.V;
+    w.T_ = p.T; w.ngroups = c.ngroups; w.nwe = (int)p.wall_elems.size();
+    tbegin(c, 1);
+    hipLaunchKernelGGL(k_wall<T>, dim3(c.nblk_wall), dim3(256), 0, c.stream, w);
+    HIPCK(hipGetLastError());
+    tend(c, 1);
+  }
+  c.cur = 1 - c.cur;
+  tbegin(c, 2);
+  int st = launch_moments<T>(c);
+  if (st) return st;
+  tend(c, 2);
+  if (c.itime % 1000 == 0) {
+    tbegin(c, 5);
+    st = launch_points(c, 1);
+    if (st) return st;
+    tend(c, 5);
+  }
+  tbegin(c, 4);
+  st = launch_final(c, record);
+  tend(c, 4);
+  return st;
+}
+
+template <typename T>
+int upload_tables(Ctx& c) {
+  const Prep& p = c.p;
+  const int E = p.E, V = p.V;
+  const size_t N3 = 3 * (size_t)E;
+  std::vector<T> geo(8 * (size_t)E), enx(3 * (size_t)E), eny(3 * (size_t)E), eal(3 * (size_t)E);
+  std::vector<int> einfo(3 * (size_t)E), eslot(3 * (size_t)E), emask(E), ewslot(E);
+  for (int e = 0; e < E; ++e) {
+    const double* g = &p.geome[7 * (size_t)e];
+    for (int k = 0; k < 6; ++k) geo[(size_t)k * E + e] = (T)g[k];
+    geo[6 * (size_t)E + e] = (T)(0.5 * g[6]);
+    geo[7 * (size_t)E + e] = (T)p.mmat[3 * (size_t)e];
+    for (int k = 0; k < 3; ++k) {
+      const ElemEdge& ed = p.edges[3 * (size_t)e + k];
+      const size_t i = (size_t)k * E + e;
+      einfo[i] = einfo_pack(ed.kind, ed.a1, ed.a2, ed.b1 < 0 ? 0 : ed.b1,
+                            ed.b2 < 0 ? 0 : ed.b2, ed.sign > 0 ? 1 : 0);
+      eslot[i] = ed.kind == kInterior ? p.nbr_slot[3 * (size_t)e + k]
+                                      : (ed.kind == kWall ? ed.wside : -1);
+      enx[i] = (T)ed.nx;
+      eny[i] = (T)ed.ny;
+      eal[i] = (T)ed.alen;
+    }
+    emask[e] = p.inflow_mask[e];
+    ewslot[e] = p.wall_elem_slot[e];
+  }
+  int st;
+  if ((st = upload((T**)&c.geo, geo, c.owned))) return st;
+  if ((st = upload((T**)&c.enx, enx, c.owned))) return st;
+  if ((st = upload((T**)&c.eny, eny, c.owned))) return st;
+  if ((st = upload((T**)&c.ealen, eal, c.owned))) return st;
+  if ((st = upload(&c.einfo, einfo, c.owned))) return st;
+  if ((st = upload(&c.eslot, eslot, c.owned))) return st;
+  if ((st = upload(&c.emask, emask, c.owned))) return st;
+  if ((st = upload(&c.ewslot, ewslot, c.owned))) return st;
+  std::vector<T> vtab(3 * (size_t)V);
+  std::vector<double> vtd(5 * (size_t)V);
+  for (int v = 0; v < V; ++v) {
+    vtab[v] = (T)p.cxa[v];
+    vtab[V + v] = (T)p.cya[v];
+    vtab[2 * V + v] = (T)p.finf[v];
+    vtd[v] = p.cxa[v];
+    vtd[V + v] = p.cya[v];
+    vtd[2 * V + v] = p.cxb[v];
+    vtd[3 * V + v] = p.cyb[v];
+    vtd[4 * V + v] = p.wv[v];
+  }
+  if ((st = upload((T**)&c.vtab, vtab, c.owned))) return st;
+  if ((st = upload(&c.vtd, vtd, c.owned))) return st;
+  const size_t NW = p.wall_sides.size();
+  std::vector<int> ri(2 * NW * V);
+  std::vector<T> rt(3 * NW * V);
+  for (size_t i = 0; i < NW * V; ++i) {
+    ri[2 * i] = p.refl_i1[i];
+    ri[2 * i + 1] = p.refl_i2[i];
+    rt[3 * i] = (T)p.refl_zd1[i];
+    rt[3 * i + 1] = (T)p.refl_zd2[i];
+    rt[3 * i + 2] = (T)p.refl_mw[i];
+  }
+  if ((st = upload(&c.refl_i, ri, c.owned))) return st;
+  if ((st = upload((T**)&c.refl_t, rt, c.owned))) return st;
+  if ((st = upload(&c.wall_bside, p.wall_sides, c.owned))) return st;
+  std::vector<int> tiles(4 * p.tile_e0.size());
+  for (size_t t = 0; t < p.tile_e0.size(); ++t) {
+    tiles[4 * t] = p.tile_e0[t];
+    tiles[4 * t + 1] = p.tile_nown[t];
+    tiles[4 * t + 2] = p.tile_hoff[t];
+    tiles[4 * t + 3] = p.tile_nhalo[t];
+  }
+  if ((st = upload(&c.tiles, tiles, c.owned))) return st;
+  if ((st = upload(&c.halo, p.halo_elems, c.owned))) return st;
+  if ((st = upload(&c.wall_elems, p.wall_elems, c.owned))) return st;
+  std::vector<int> es(4 * p.eta_side.size());
+  std::vector<double> eg(3 * p.eta_side.size());
+  for (size_t j = 0; j < p.eta_side.size(); ++j) {
+    es[4 * j] = p.eta_side[j];
+    es[4 * j + 1] = p.eta_elem[j];
+    es[4 * j + 2] = p.eta_in1[j];
+    es[4 * j + 3] = p.eta_in2[j];
+    eg[3 * j] = p.eta_nx[j];
+    eg[3 * j + 1] = p.eta_ny[j];
+    eg[3 * j + 2] = p.eta_bottom[j];
+  }
+  if ((st = upload(&c.eta_side, es, c.owned))) return st;
+  if ((st = upload(&c.eta_geo, eg, c.owned))) return st;
+  std::vector<int> fs(2 * p.force_p1.size());
+  for (size_t j = 0; j < p.force_p1.size(); ++j) {
+    fs[2 * j] = p.force_p1[j];
+    fs[2 * j + 1] = p.force_p2[j];
+  }
+  if ((st = upload(&c.fside, fs, c.owned))) return st;
+  std::vector<double> rj(E);
+  for (int e = 0; e < E; ++e) rj[e] = p.geome[7 * (size_t)e + 6];
+  if ((st = upload(&c.rj, rj, c.owned))) return st;
+  if ((st = upload(&c.pt_ptr, p.pt_ptr, c.owned))) return st;
+  if ((st = upload(&c.pt_node, p.pt_node, c.owned))) return st;
+  if ((st = upload(&c.coord, p.coord, c.owned))) return st;
+  if ((st = upload(&c.intma, p.intma, c.owned))) return st;
+  std::vector<double> eta0(std::max(p.B, 1), 0.0);
+  if ((st = upload(&c.eta, eta0, c.owned))) return st;
+  if ((st = alloc((T**)&c.np, 5 * N3, c.owned))) return st;
+  if ((st = alloc(&c.mom, 5 * N3, c.owned))) return st;
+  if ((st = alloc(&c.pts, 6 * (size_t)p.P, c.owned))) return st;
+  if ((st = alloc(&c.ps_glob, (size_t)p.P, c.owned))) return st;
+  if ((st = alloc(&c.ps4, 4 * (size_t)p.P, c.owned))) return st;
+  if ((st = alloc((T**)&c.wpart, c.gs ? p.wall_elems.size() * (size_t)V * 6 : 1, c.owned)))
+    return st;
+  for (int b = 0; b < 2; ++b)
+    if ((st = alloc((T**)&c.f[b], (size_t)V * N3, c.owned))) return st;
+  return kOk;
+}
+
+template <typename T>
+int seed_points(Ctx& c, const std::vector<double>& ps4) {
+  HIPCK(hipMemcpyAsync(c.ps4, ps4.data(), ps4.size() * sizeof(double), hipMemcpyHostToDevice,
+                       c.stream));
+  const size_t N3 = 3 * (size_t)c.p.E;
+  hipLaunchKernelGGL(k_seed<T>, dim3((unsigned)((N3 + 255) / 256), c.p.V), dim3(256), 0,
+                     c.stream, (T*)c.f[c.cur], c.intma, c.ps4, c.vtd, c.p.E, c.p.V, c.p.P);
+  HIPCK(hipGetLastError());
+  return kOk;
+}
+
+// after f changes outside a step: initial MACROS + GETMAC + first DTE
+template <typename T>
+int refresh(Ctx& c) {
+  int st = launch_moments<T>(c);
+  if (st) return st;
+  if ((st = launch_points(c, 1))) return st;
+  // DTE for the next step, no record
+  HIPCK(hipMemsetAsync(c.res_part, 0, sizeof(double) * (c.nblk_upd + c.nblk_wall), c.stream));
+  return launch_final(c, -1);
+}
+
+int do_refresh(Ctx& c) { return c.prec ? refresh<float>(c) : refresh<double>(c); }
+
+}  // namespace
+}  // namespace dgbgk
+
+using namespace dgbgk;
+
+struct dgbgk_ctx {
+  Ctx c;
+};
+struct dgbgk_setup {
+  Prep p;
+};
+
+static Params to_params(const dgbgk_params* prm) {
+  Params q;
+  q.rv = prm->rv; q.csafm = prm->csafm; q.alpha = prm->alpha; q.rgas = prm->rgas;
+  q.d = prm->d; q.m = prm->m; q.t1 = prm->t1; q.p1 = prm->p1; q.u0 = prm->u0; q.v0 = prm->v0;
+  for (int k = 0; k < 4; ++k) q.cinf[k] = prm->cinf[k];
+  q.inf = prm->inf; q.immat = prm->immat; q.precision = prm->precision;
+  q.lookup = prm->lookup; q.order = prm->order; q.vchunk = prm->vchunk;
+  return q;
+}
+
+extern "C" {
+
+const char* dgbgk_last_error(void) { return g_err.c_str(); }
+
+int dgbgk_setup_create(int nelem, int npoin, int nboun, const int* intma, const double* coord,
+                       const int* bsido, int rorder, int torder, const double* eta,
+                       const double* w, const dgbgk_params* prm, dgbgk_setup** out) {
+  if (!out || !prm || !intma || !coord || (!bsido && nboun) || !eta || !w)
+    return fail(kErrArg, "null argument");
+  *out = nullptr;
+  auto* s = new (std::nothrow) dgbgk_setup;
+  if (!s) return fail(kErrNoMem, "out of host memory");
+  std::string msg;
+  int st = prepare(nelem, npoin, nboun, intma, coord, bsido, rorder, torder, eta, w,
+                   to_params(prm), kSlots, s->p, msg);
+  if (st) {
+    delete s;
+    return fail(st, msg);
+  }
+  *out = s;
+  return kOk;
+}
+
+int dgbgk_setup_sizes(dgbgk_setup* s, int* nside, int* nvel, int* ntiles, int* nwall_sides,
+                      int* ring_separable) {
+  if (!s) return fail(kErrArg, "null setup");
+  if (nside) *nside = s->p.S;
+  if (nvel) *nvel = s->p.V;
+  if (ntiles) *ntiles = (int)s->p.tile_e0.size();
+  if (nwall_sides) *nwall_sides = (int)s->p.wall_sides.size();
+  if (ring_separable) *ring_separable = s->p.ring_separable ? 1 : 0;
+  return kOk;
+}
+
+static void copy_tables(const Prep& p, int* iside, double* nx, double* ny, double* el,
+                        double* geome, double* relen, double* mmat, double* rsido,
+                        double* vcord, double* sumweight, double* dtec) {
+  auto cp = [](auto* dst, const auto& v) {
+    if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+  };
+  cp(iside, p.iside); cp(nx, p.nx); cp(ny, p.ny); cp(el, p.el); cp(geome, p.geome);
+  cp(relen, p.relen); cp(mmat, p.mmat); cp(rsido, p.rsido); cp(vcord, p.vcord);
+  if (sumweight) *sumweight = p.sumweight;
+  if (dtec) *dtec = p.dtec;
+}
+
+int dgbgk_setup_tables_host(dgbgk_setup* s, int* iside, double* nx, double* ny, double* el,
+                            double* geome, double* relen, double* mmat, double* rsido,
+                            double* vcord, double* sumweight, double* dtec) {
+  if (!s) return fail(kErrArg, "null setup");
+  copy_tables(s->p, iside, nx, ny, el, geome, relen, mmat, rsido, vcord, sumweight, dtec);
+  return kOk;
+}
+
+void dgbgk_setup_destroy(dgbgk_setup* s) { delete s; }
+
+int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double* coord,
+                 const int* bsido, int rorder, int torder, const double* eta, const double* w,
+                 const dgbgk_params* prm, dgbgk_ctx** out) {
+  if (!out || !prm || !intma || !coord || (!bsido && nboun) || !eta || !w)
+    return fail(kErrArg, "null argument");
+  *out = nullptr;
+  if (prm->precision != DGBGK_FP64 && prm->precision != DGBGK_FP32)
+    return fail(kErrArg, "precision must be DGBGK_FP64 or DGBGK_FP32");
+  auto* h = new (std::nothrow) dgbgk_ctx;
+  if (!h) return fail(kErrNoMem, "out of host memory");
+  Ctx& c = h->c;
+  std::string msg;
+  int st = prepare(nelem, npoin, nboun, intma, coord, bsido, rorder, torder, eta, w,
+                   to_params(prm), kSlots, c.p, msg);
+  if (st) {
+    delete h;
+    return fail(st, msg);
+  }
+  c.prec = prm->precision;
+  c.gs = prm->order == DGBGK_ORDER_REFERENCE ? 1 : 0;
+  c.tsz = c.prec ? 4 : 8;
+  const Prep& p = c.p;
+  auto cleanup = [&](int code) {
+    for (void* q : c.owned) hipFree(q);
+    if (c.stream) hipStreamDestroy(c.stream);
+    delete h;
+    return code;
+  };
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return cleanup(fail(kErrHip, "no HIP device available"));
+  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(kErrHip, "hipStreamCreate failed"));
+  // launch geometry
+  c.ntiles = (int)p.tile_e0.size();
+  int vch = prm->vchunk;
+  if (vch <= 0) {
+    const int want = 4096;  // blocks: 16 per CU on 256 CUs
+    int nv = std::max(1, std::min(p.V, (want + c.ntiles - 1) / c.ntiles));
+    vch = (p.V + nv - 1) / nv;
+  }
+  c.vchunk = std::max(1, std::min(vch, p.V));
+  c.nvchunks = (p.V + c.vchunk - 1) / c.vchunk;
+  c.nblk_upd = c.ntiles * c.nvchunks;
+  c.ngroups = p.ring_separable ? p.R : 1;
+  const int nwe = (int)p.wall_elems.size();
+  c.nblk_wall = (c.gs && nwe) ? (nwe * c.ngroups + 255) / 256 : 0;
+  c.nblk_mom = (int)((3 * (size_t)p.E + 255) / 256);
+  c.neta = (int)p.eta_side.size();
+  c.nforce = (int)p.force_p1.size();
+  st = c.prec ? upload_tables<float>(c) : upload_tables<double>(c);
+  if (st) return cleanup(st);
+  if ((st = alloc(&c.res_part, (size_t)c.nblk_upd + c.nblk_wall, c.owned))) return cleanup(st);
+  if ((st = alloc(&c.mu_part, (size_t)c.nblk_mom, c.owned))) return cleanup(st);
+  if ((st = alloc(&c.mass_part, (size_t)c.nblk_mom, c.owned))) return cleanup(st);
+  std::vector<double> scal(8, 0.0);
+  scal[4] = p.dtec;
+  if ((st = upload(&c.scal, scal, c.owned))) return cleanup(st);
+  c.rec_cap = 1024;
+  if ((st = alloc(&c.rec, (size_t)c.rec_cap, c.owned))) return cleanup(st);
+  // INICON2 (src/inicon2.f90:120-173): uniform Maxwellian from T1, P1, U0, V0
+  {
+    const double t1 = prm->t1, p1 = prm->p1, gc = prm->rgas, m = prm->m;
+    const double rho1 = (p1 * 100000.0) / (gc * t1);
+    const double n1 = rho1 * kNa / m;
+    const double tmp1 = rho1 / (2 * p1 * 100000.0);
+    const double beta1 = std::sqrt(tmp1);
+    std::vector<double> ps4(4 * (size_t)p.P);
+    for (int q = 0; q < p.P; ++q) {
+      ps4[q] = n1 >= 1e-20 ? n1 : 0.0;
+      ps4[p.P + q] = beta1 * beta1;
+      ps4[2 * (size_t)p.P + q] = prm->u0;
+      ps4[3 * (size_t)p.P + q] = prm->v0;
+    }
+    st = c.prec ? seed_points<float>(c, ps4) : seed_points<double>(c, ps4);
+    if (st) return cleanup(st);
+  }
+  if ((st = do_refresh(c))) return cleanup(st);
+  if (hipStreamSynchronize(c.stream) != hipSuccess)
+    return cleanup(fail(kErrHip, "initialisation failed on the device"));
+  *out = h;
+  return kOk;
+}
+
+void dgbgk_destroy(dgbgk_ctx* h) {
+  if (!h) return;
+  Ctx& c = h->c;
+  if (c.stream) hipStreamSynchronize(c.stream);
+  for (auto& e : c.ev_pool) {
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  for (void* q : c.owned) hipFree(q);
+  if (c.stream) hipStreamDestroy(c.stream);
+  delete h;
+}
+
+int dgbgk_step(dgbgk_ctx* h, int nsteps, dgbgk_step_info* info) {
+  if (!h || nsteps < 0) return fail(kErrArg, "bad arguments");
+  Ctx& c = h->c;
+  int done = 0;
+  while (done < nsteps) {
+    const int batch = std::min(nsteps - done, c.rec_cap);
+    for (int s = 0; s < batch; ++s) {
+      int st = c.prec ? step_once<float>(c, s) : step_once<double>(c, s);
+      if (st) return st;
+    }
+    if (info) {
+      HIPCK(hipMemcpyAsync(info + done, c.rec, batch * sizeof(StepRec), hipMemcpyDeviceToHost,
+                           c.stream));
+    }
+    HIPCK(hipStreamSynchronize(c.stream));
+    tharvest(c);
+    done += batch;
+  }
+  return kOk;
+}
+
+int dgbgk_get_f(dgbgk_ctx* h, double* f) {
+  if (!h || !f) return fail(kErrArg, "null argument");
+  Ctx& c = h->c;
+  const size_t n = 3 * (size_t)c.p.E * c.p.V;
+  double* d = nullptr;
+  HIPCK(hipMalloc((void**)&d, n * sizeof(double)));
+  const size_t N3 = 3 * (size_t)c.p.E;
+  dim3 g((unsigned)((N3 + 255) / 256), c.p.V);
+  if (c.prec)
+    hipLaunchKernelGGL(k_to_host<float>, g, dim3(256), 0, c.stream, (const float*)c.f[c.cur], d,
+                       c.p.E, c.p.V);
+  else
+    hipLaunchKernelGGL(k_to_host<double>, g, dim3(256), 0, c.stream,
+                       (const double*)c.f[c.cur], d, c.p.E, c.p.V);
+  hipError_t e1 = hipMemcpyAsync(f, d, n * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  hipError_t e2 = hipStreamSynchronize(c.stream);
+  hipFree(d);
+  if (e1 != hipSuccess || e2 != hipSuccess) return fail(kErrHip, "get_f failed");
+  return kOk;
+}
+
+int dgbgk_set_f(dgbgk_ctx* h, const double* f) {
+  if (!h || !f) return fail(kErrArg, "null argument");
+  Ctx& c = h->c;
+  const size_t n = 3 * (size_t)c.p.E * c.p.V;
+  double* d = nullptr;
+  HIPCK(hipMalloc((void**)&d, n * sizeof(double)));
+  hipError_t e1 = hipMemcpyAsync(d, f, n * sizeof(double), hipMemcpyHostToDevice, c.stream);
+  const size_t N3 = 3 * (size_t)c.p.E;
+  dim3 g((unsigned)((N3 + 255) / 256), c.p.V);
+  if (c.prec)
+    hipLaunchKernelGGL(k_to_device<float>, g, dim3(256), 0, c.stream, d, (float*)c.f[c.cur],
+                       c.p.E, c.p.V);
+  else
+    hipLaunchKernelGGL(k_to_device<double>, g, dim3(256), 0, c.stream, d,
+                       (double*)c.f[c.cur], c.p.E, c.p.V);
+  int st = do_refresh(c);
+  hipError_t e2 = hipStreamSynchronize(c.stream);
+  hipFree(d);
+  if (st) return st;
+  if (e1 != hipSuccess || e2 != hipSuccess) return fail(kErrHip, "set_f failed");
+  return kOk;
+}
+
+int dgbgk_seed_maxwellian(dgbgk_ctx* h, const double* rho, const double* u, const double* v,
+                          const double* temp) {
+  if (!h || !rho || !u || !v || !temp) return fail(kErrArg, "null argument");
+  Ctx& c = h->c;
+  const Prep& p = c.p;
+  std::vector<double> ps4(4 * (size_t)p.P);
+  for (int q = 0; q < p.P; ++q) {
+    // INICON2 with point state: n = rho*NA/M, beta^2 = rho/(2p), p = rho*R*T
+    const double pr = rho[q] * p.prm.rgas * temp[q];
+    const double n1 = rho[q] * kNa / p.prm.m;
+    ps4[q] = (n1 >= 1e-20 && pr > 0.0) ? n1 : 0.0;
+    ps4[p.P + q] = pr > 0.0 ? rho[q] / (2 * pr) : 0.0;
+    ps4[2 * (size_t)p.P + q] = u[q];
+    ps4[3 * (size_t)p.P + q] = v[q];
+  }
+  int st = c.prec ? seed_points<float>(c, ps4) : seed_points<double>(c, ps4);
+  if (st) return st;
+  if ((st = do_refresh(c))) return st;
+  HIPCK(hipStreamSynchronize(c.stream));
+  return kOk;
+}
+
+int dgbgk_node_fields(dgbgk_ctx* h, double* disnd, double* disux, double* disuy,
+                      double* disps, double* mu) {
+  if (!h) return fail(kErrArg, "null context");
+  Ctx& c = h->c;
+  const size_t N3 = 3 * (size_t)c.p.E;
+  double* dst[5] = {disnd, disux, disuy, disps, mu};
+  for (int i = 0; i < 5; ++i)
+    if (dst[i])
+      HIPCK(hipMemcpyAsync(dst[i], c.mom + i * N3, N3 * sizeof(double), hipMemcpyDeviceToHost,
+                           c.stream));
+  HIPCK(hipStreamSynchronize(c.stream));
+  return kOk;
+}
+
+int dgbgk_point_fields(dgbgk_ctx* h, double* nd, double* rho, double* uvel, double* vvel,
+                       double* ps, double* temp) {
+  if (!h) return fail(kErrArg, "null context");
+  Ctx& c = h->c;
+  int st = launch_points(c, 0);
+  if (st) return st;
+  const size_t P = c.p.P;
+  double* dst[6] = {nd, rho, uvel, vvel, ps, temp};
+  for (int i = 0; i < 6; ++i)
+    if (dst[i])
+      HIPCK(hipMemcpyAsync(dst[i], c.pts + i * P, P * sizeof(double), hipMemcpyDeviceToHost,
+                           c.stream));
+  HIPCK(hipStreamSynchronize(c.stream));
+  return kOk;
+}
+
+int dgbgk_forces(dgbgk_ctx* h, double* lift, double* drag) {
+  if (!h) return fail(kErrArg, "null context");
+  Ctx& c = h->c;
+  const Prep& p = c.p;
+  std::vector<double> ps(p.P);
+  int st = dgbgk_point_fields(h, nullptr, nullptr, nullptr, nullptr, ps.data(), nullptr);
+  if (st) return st;
+  double l = 0.0, d = 0.0;
+  for (size_t j = 0; j < p.force_p1.size(); ++j) {
+    const int p1 = p.force_p1[j], p2 = p.force_p2[j];
+    const double x1 = p.coord[2 * (size_t)p1], y1 = p.coord[2 * (size_t)p1 + 1];
+    const double x2 = p.coord[2 * (size_t)p2], y2 = p.coord[2 * (size_t)p2 + 1];
+    const double len = std::sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
+    const double px = -(y2 - y1), py = x2 - x1, mag = std::sqrt(px * px + py * py);
+    const double psav = 0.5 * (ps[p1] + ps[p2]);
+    l -= psav * len * (py / mag);
+    d -= psav * len * (px / mag);
+  }
+  if (lift) *lift = l;
+  if (drag) *drag = d;
+  return kOk;
+}
+
+int dgbgk_wall_eta(dgbgk_ctx* h, double* eta) {
+  if (!h || !eta) return fail(kErrArg, "null argument");
+  Ctx& c = h->c;
+  HIPCK(hipMemcpyAsync(eta, c.eta, std::max(c.p.B, 0) * sizeof(double), hipMemcpyDeviceToHost,
+                       c.stream));
+  HIPCK(hipStreamSynchronize(c.stream));
+  return kOk;
+}
+
+int dgbgk_sizes(dgbgk_ctx* h, int* nside, int* nvel, int* nelem, int* npoin, int* nboun,
+                int* itime) {
+  if (!h) return fail(kErrArg, "null context");
+  const Ctx& c = h->c;
+  if (nside) *nside = c.p.S;
+  if (nvel) *nvel = c.p.V;
+  if (nelem) *nelem = c.p.E;
+  if (npoin) *npoin = c.p.P;
+  if (nboun) *nboun = c.p.B;
+  if (itime) *itime = c.itime;
+  return kOk;
+}
+
+int dgbgk_setup_tables(dgbgk_ctx* h, int* iside, double* nx, double* ny, double* el,
+                       double* geome, double* relen, double* mmat, double* rsido, double* vcord,
+                       double* sumweight, double* dtec) {
+  if (!h) return fail(kErrArg, "null context");
+  copy_tables(h->c.p, iside, nx, ny, el, geome, relen, mmat, rsido, vcord, sumweight, dtec);
+  return kOk;
+}
+
+int dgbgk_timing(dgbgk_ctx* h, int enable) {
+  if (!h) return fail(kErrArg, "null context");
+  Ctx& c = h->c;
+  HIPCK(hipStreamSynchronize(c.stream));
+  tharvest(c);
+  c.timing = enable != 0;
+  for (int k = 0; k < 8; ++k) {
+    c.t_ms[k] = 0;
+    c.t_cnt[k] = 0;
+  }
+  return kOk;
+}
+
+int dgbgk_kernel_time(dgbgk_ctx* h, int kernel, double* ms_total, long* count) {
+  if (!h || kernel < 0 || kernel >= 8) return fail(kErrArg, "bad arguments");
+  HIPCK(hipStreamSynchronize(h->c.stream));
+  tharvest(h->c);
+  if (ms_total) *ms_total = h->c.t_ms[kernel];
+  if (count) *count = h->c.t_cnt[kernel];
+  return kOk;
+}
+
+int dgbgk_sync(dgbgk_ctx* h) {
+  if (!h) return fail(kErrArg, "null context");
+  HIPCK(hipStreamSynchronize(h->c.stream));
+  return kOk;
+}
+
+int dgbgk_dof_bytes(dgbgk_ctx* h, int* bytes_per_value) {
+  if (!h) return fail(kErrArg, "null context");
+  if (bytes_per_value) *bytes_per_value = (int)h->c.tsz;
+  return kOk;
+}
+
+}  // extern "C"

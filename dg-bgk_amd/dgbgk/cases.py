@@ -1,0 +1,131 @@
+"""Workloads as reference run decks: mesh + radial quadrature + deck keys.
+
+Flow conditions are non-dimensionalised through the reference's own deck
+keys (src/spectralbgk.f90:116-147): gas constant R = 1 and molar mass
+M = 6.022e26 make the molecular mass M/NA exactly 1, so number density is
+mass density; T = 1 and P = 1e-5 bar (P enters multiplied by 10**5) give
+rho = p = 1.  The thermal speed sqrt(RT) is then 1, the 2-D monatomic sound
+speed is sqrt(gamma R T) with gamma = 2, and Kn enters through the
+hard-sphere diameter d = (sqrt(2) * 3.1416 * n * Kn * L)**-0.5 that makes the
+reference's collision frequency (src/compmu.f90:295,341-342) give a mean
+free path of Kn * L.
+
+Radial quadrature: R speeds uniform on (0, rv] at cell midpoints,
+eta_i = (2i-1)/R - 1 with weights 2/R.  The weights sum to 2, which the
+reference's fixed normalisation FRAC = 4/SUMWEIGHT (src/macros.f90:55)
+requires for a Maxwellian to keep its density; end-point trapezoid weights
+on (0, rv] sum to 2 - 1/R and would inflate every moment by 2R/(2R-1).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import meshgen
+
+GAMMA_2D = 2.0
+PI_REF = 3.1416
+
+
+@dataclass
+class Case:
+    name: str
+    mesh: meshgen.Mesh
+    eta: np.ndarray          # radial nodes in [-1, 1]
+    w: np.ndarray            # radial weights
+    torder: int
+    prm: dict = field(default_factory=dict)
+    nsteps: int = 200
+
+    @property
+    def rorder(self) -> int:
+        return int(len(self.eta))
+
+    @property
+    def nvel(self) -> int:
+        return self.rorder * self.torder
+
+    @property
+    def dofv_per_step(self) -> int:
+        return 3 * self.mesh.nelem * self.nvel
+
+
+def midpoint_speeds(r: int):
+    i = np.arange(1, r + 1, dtype=np.float64)
+    return (2 * i - 1) / r - 1.0, np.full(r, 2.0 / r)
+
+
+def diameter_for_kn(kn: float, length: float = 1.0, n: float = 1.0) -> float:
+    return (math.sqrt(2.0) * PI_REF * n * kn * length) ** -0.5
+
+
+def base_params(u0: float, kn: float, length: float, rv: float, alpha: float,
+                csafm: float = 0.5, v0: float = 0.0) -> dict:
+    return dict(rv=rv, csafm=csafm, alpha=alpha, rgas=1.0, d=diameter_for_kn(kn, length),
+                m=6.022e26, t1=1.0, p1=1e-5, u0=u0, v0=v0,
+                cinf=[1.0, 1e-5, u0, v0], inf=1)
+
+
+def mach_to_u(mach: float) -> float:
+    return mach * math.sqrt(GAMMA_2D)
+
+
+def rv_for(u0: float) -> float:
+    """Velocity-grid radius: 4 thermal speeds beyond the flow speed."""
+    return math.ceil(2.0 * (4.0 + abs(u0))) / 2.0
+
+
+def config1(n: int = 32, rorder: int = 8, torder: int = 16) -> Case:
+    """Unit square, 2n^2 P1 elements; 16 angles x 8 speeds on (0,4];
+    rho=1, T=1, u=(0.1,0); inflow left, outflow right, specular walls top
+    and bottom (type 3, alpha=0); Kn=0.1; 200 steps at CFL 0.5."""
+    eta, w = midpoint_speeds(rorder)
+    m = meshgen.unit_square(n, left=1, right=2, bottom=3, top=3, tw=1, alpha100=0)
+    prm = base_params(u0=0.1, kn=0.1, length=1.0, rv=4.0, alpha=0.0)
+    return Case("c1_unit_square", m, eta, w, torder, prm, nsteps=200)
+
+
+def cylinder_mesh(target_elems: int = 100_000, seed: int = 1) -> meshgen.Mesh:
+    """20D x 10D box, cylinder of diameter D=1 at (5,5), graded Delaunay."""
+    hmax = 0.068 * math.sqrt(100_000 / target_elems)
+    h0 = min(0.02 * math.sqrt(100_000 / target_elems), 0.5 * hmax)
+    return meshgen.body_in_box(20.0, 10.0, 5.0, 5.0, 1.0, h0=h0, growth=0.12,
+                               hmax=hmax, seed=seed, wall_type=4, tw=1,
+                               alpha100=100, far_types=(1, 2, 2, 2),
+                               name=f"cylinder_{target_elems}")
+
+
+def config2(target_elems: int = 100_000, rorder: int = 16, torder: int = 32,
+            mach: float = 0.5, kn: float = 0.05) -> Case:
+    """Cylinder flow: Mach 0.5, Kn 0.05 (on D), diffuse isothermal wall T=1,
+    512 ordinates, 2000 steps."""
+    m = cylinder_mesh(target_elems)
+    u0 = mach_to_u(mach)
+    eta, w = midpoint_speeds(rorder)
+    prm = base_params(u0=u0, kn=kn, length=1.0, rv=rv_for(u0), alpha=1.0)
+    return Case("c2_cylinder", m, eta, w, torder, prm, nsteps=2000)
+
+
+def config5(target_elems: int = 1_000_000, rorder: int = 32, torder: int = 64,
+            mach: float = 0.3, kn: float = 1.0) -> Case:
+    """Particle of diameter D in a 200D x 100D box, ~1M graded triangles,
+    2048 ordinates, Mach 0.3, Kn 1."""
+    hmax = 0.215 * math.sqrt(1_000_000 / target_elems)
+    m = meshgen.body_in_box(200.0, 100.0, 60.0, 50.0, 1.0, h0=min(0.02, hmax / 4),
+                            growth=0.1, hmax=hmax, seed=5, wall_type=4, tw=1,
+                            alpha100=100, far_types=(1, 2, 2, 2),
+                            name=f"particle_{target_elems}")
+    u0 = mach_to_u(mach)
+    eta, w = midpoint_speeds(rorder)
+    prm = base_params(u0=u0, kn=kn, length=1.0, rv=rv_for(u0), alpha=1.0)
+    return Case("c5_particle", m, eta, w, torder, prm, nsteps=100)
+
+
+def lobatto_case(mesh: meshgen.Mesh, eta, w, torder: int, alpha: float = 0.9,
+                 u0: float = 0.3, kn: float = 0.1, rv: float = 5.0) -> Case:
+    """Any mesh with a given radial quadrature (e.g. a Lobatto file)."""
+    prm = base_params(u0=u0, kn=kn, length=1.0, rv=rv, alpha=alpha)
+    return Case("custom", mesh, np.asarray(eta, float), np.asarray(w, float),
+                torder, prm)

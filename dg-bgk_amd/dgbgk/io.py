@@ -1,0 +1,167 @@
+"""The reference's file formats: run deck (.con), mesh (.plt), radial
+quadrature (Lobatto file) and the output files RESIDUAL.RES, RESULTS1.RES,
+RESULTS2.RES and FILE.OUT.
+
+Readers accept what the reference's list-directed READs accept for these
+files; writers produce the reference's layouts (src/advnce.f90:429-432,
+488-489; src/spectralbgk.f90:240-282).
+"""
+from __future__ import annotations
+
+import os
+import re
+
+import numpy as np
+
+from .meshgen import Mesh
+
+# Defaults of the /Input/ namelist (src/spectralbgk.f90:116-147).
+DECK_DEFAULTS = {
+    "TORDER": 20, "NTIME": 10000, "FORCEOUT": 0, "IMMAT": 1, "INF": 1, "RS": 0,
+    "CSAFM": 0.5, "RV": 2000.0, "T1": 293.0, "P1": 0.1, "U0": 100.0, "V0": 0.0,
+    "W": 1.0, "ALPHA": 0.9, "R": 287.0, "D": 250e-12, "M": 32.0,
+    "CINF": [294.0, 0.001473, 171.85, 0.0], "NVSPACEPART": 1,
+    "LOBATTOFILE": "Lobatto20.txt", "PSPACEFILE": "AEROFOIL.RES", "OUTFILE": "FILE.OUT",
+    "PARTITIONFILE": "METIS3WAYPARTAEROFOIL13.RES", "RESTARTINFILE": "RESTART.RES",
+    "RESIDUALFILE": "RESIDUAL.RES", "RESULTSFILE1": "RESULTS1.RES",
+    "RESULTSFILE2": "RESULTS2.RES", "RESTARTOUTFILE": "RESTART.RES",
+    "GIDMESHFILE": "GIDMESH.RES",
+}
+
+
+def _fnum(tok: str) -> float:
+    return float(tok.replace("d", "e").replace("D", "e"))
+
+
+def read_deck(path: str) -> dict:
+    """Parses the &Input ... / namelist of IVD%KEY = value entries."""
+    deck = {k: (list(v) if isinstance(v, list) else v) for k, v in DECK_DEFAULTS.items()}
+    text = open(path).read()
+    m = re.search(r"&\s*input(.*?)(^\s*/|/\s*$)", text, re.I | re.S | re.M)
+    body = m.group(1) if m else text
+    body = "\n".join(line.split("!")[0] for line in body.splitlines())
+    for mm in re.finditer(r"IVD%\s*([A-Za-z0-9_]+)\s*(\(\s*(\d+)\s*\))?\s*=\s*"
+                          r"(\"[^\"]*\"|'[^']*'|[^,\n]+)", body, re.I):
+        key, idx, val = mm.group(1).upper(), mm.group(3), mm.group(4).strip()
+        if val[:1] in "\"'":
+            v = val[1:-1]
+        else:
+            v = val.rstrip(",").strip()
+            try:
+                v = _fnum(v)
+            except ValueError:
+                pass
+        if idx is not None:
+            deck[key][int(idx) - 1] = float(v)
+        elif key in deck and isinstance(deck[key], int) and isinstance(v, float):
+            deck[key] = int(v)
+        else:
+            deck[key] = v
+    return deck
+
+
+def read_lobatto(path: str):
+    """Line 1: RORDER; then RORDER lines 'eta weight' (src/vspace.f90:52-54)."""
+    toks = open(path).read().split()
+    r = int(float(toks[0]))
+    vals = np.array([_fnum(t) for t in toks[1:1 + 2 * r]], dtype=np.float64).reshape(r, 2)
+    return vals[:, 0].copy(), vals[:, 1].copy()
+
+
+def write_lobatto(path: str, eta, w):
+    with open(path, "w") as fh:
+        fh.write(f"{len(eta)}\n")
+        for a, b in zip(eta, w):
+            fh.write(f"{a!r}\t{b!r}\n")
+
+
+def read_plt(path: str) -> Mesh:
+    """Mesh file read as in src/spectralbgk.f90:240-272 and
+    src/gtinpt.f90:113-139: NLINES, NLINES title lines, a text line, NDIMN,
+    a text line, 'NELEM NPOIN NBOUN', then the connectivity, coordinate and
+    boundary-side blocks, each after one header line."""
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    it = iter(lines)
+
+    def nxt():
+        return next(it)
+
+    nlines = int(nxt().split()[0])
+    for _ in range(nlines):
+        nxt()
+    nxt()
+    ndimn = int(nxt().split()[0])
+    if ndimn != 2:
+        raise ValueError("only 2-D meshes")
+    nxt()
+    ne, npn, nb = (int(x) for x in nxt().split()[:3])
+    nxt()
+    intma = np.empty((ne, 3), np.int32)
+    for i in range(ne):
+        t = nxt().split()
+        intma[i] = [int(t[1]), int(t[2]), int(t[3])]
+    nxt()
+    coord = np.empty((npn, 2))
+    for i in range(npn):
+        t = nxt().split()
+        coord[i] = [_fnum(t[1]), _fnum(t[2])]
+    nxt()
+    bsido = np.empty((nb, 6), np.int32)
+    for i in range(nb):
+        t = nxt().split()
+        bsido[i] = [int(float(x)) for x in t[:6]]
+    return Mesh(intma=intma, coord=coord, bsido=bsido, name=os.path.basename(path))
+
+
+def write_plt(path: str, mesh: Mesh, title: str = "title"):
+    with open(path, "w") as fh:
+        fh.write("           1\n")
+        fh.write(f"{title}\n")
+        fh.write("number of dimensions\n           2\n")
+        fh.write("nelem   npoin   nboun\n")
+        fh.write(f"{mesh.nelem:12d}{mesh.npoin:12d}{mesh.nboun:12d}\n")
+        fh.write("  connectivities \n")
+        for i, t in enumerate(mesh.intma):
+            fh.write(f"{i + 1:10d}{t[0]:10d}{t[1]:10d}{t[2]:10d}         0\n")
+        fh.write("  coordinates \n")
+        for i, c in enumerate(mesh.coord):
+            fh.write(f"{i + 1:10d} {c[0]!r} {c[1]!r}\n")
+        fh.write("  boundary sides \n")
+        for b in mesh.bsido:
+            fh.write(" ".join(f"{int(x):10d}" for x in b) + "\n")
+
+
+def write_deck(path: str, keys: dict):
+    """Writes an &Input namelist deck (quoted strings, as gfortran/flang need)."""
+    with open(path, "w") as fh:
+        fh.write("&Input\n")
+        for k, v in keys.items():
+            if isinstance(v, (list, tuple, np.ndarray)):
+                for i, x in enumerate(v):
+                    fh.write(f"IVD%{k}({i + 1}) = {float(x)!r},\n")
+            elif isinstance(v, str):
+                fh.write(f'IVD%{k} = "{v}",\n')
+            elif isinstance(v, (int, np.integer)):
+                fh.write(f"IVD%{k} = {int(v)},\n")
+            else:
+                fh.write(f"IVD%{k} = {float(v)!r},\n")
+        fh.write("/\n")
+
+
+def fortran_e(x: float) -> str:
+    """Fortran 1PE12.5 edit descriptor."""
+    if x == 0.0:
+        return "  0.00000E+00"
+    s = f"{x:.5E}"
+    mant, exp = s.split("E")
+    e = int(exp)
+    return f"{mant}E{'+' if e >= 0 else '-'}{abs(e):02d}".rjust(12)
+
+
+def results_block(itime: int, a, b, c) -> str:
+    """One RESULTS1/RESULTS2 block (src/advnce.f90:488-489)."""
+    out = [f"RESULTS        1   {itime:5d}    2      1      0"]
+    for ip in range(len(a)):
+        out.append(f"{ip + 1:5d}    {fortran_e(a[ip])} {fortran_e(b[ip])} {fortran_e(c[ip])} ")
+    return "\n".join(out) + "\n"

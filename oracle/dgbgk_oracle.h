@@ -1,0 +1,105 @@
+/*
+ * dgbgk_oracle.h -- FP64 CPU restatement of the DG-BGK explicit time march.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * path and the "port" CPU baseline; it is never linked into, loaded by or
+ * called from the product library.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may use it.
+ *
+ * Conventions follow the reference (DrBenEvans/DG-BGK, Fortran 90):
+ *   - integer tables hold 1-based ids, stored column-major exactly like the
+ *     Fortran arrays (INTMA(3,E) -> intma[3*(e-1)+(j-1)]);
+ *   - the distribution function uses the reference layout DISNF(3,V,E):
+ *     f[((e-1)*V + (v-1))*3 + (a-1)];
+ *   - every routine names the reference file:line it restates.
+ */
+#ifndef DGBGK_ORACLE_H
+#define DGBGK_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- mesh preprocessing (one-time setup) ---- */
+int ora_getsid(int nelem, int npoin, const int *intma, int *iside, int mxsid);
+void ora_normal(int nside, const int *iside, const double *coord,
+                double *nx, double *ny, double *el);
+void ora_getgeo(int nelem, const int *intma, const double *coord, double *geome);
+void ora_getlen(int nelem, const double *geome, double *relen);
+void ora_getmat(int nelem, const double *geome, double *mmat);
+void ora_getnor(int nboun, const int *bsido, const double *coord, double *rsido);
+void ora_vspace(int rorder, int torder, const double *eta, const double *w,
+                double *vcord, double *sumweight);
+
+/* ---- run context ---- */
+typedef struct ora_params {
+  double rv, csafm, alpha, rgas, d, m;
+  double t1, p1, u0, v0;  /* INICON2 Maxwellian seed */
+  double cinf[4];         /* inflow state (T, P[bar], U, V) */
+  int inf;                /* IVD%INF */
+  int lookup;             /* 0 = reference ring lookup, 1 = corrected */
+  int jacobi;             /* 0 = in-place velocity sweep (reference), 1 = Jacobi */
+  int faithful_scans;     /* 1 = literal O(N^2) searches of the reference */
+} ora_params;
+
+typedef struct ora_ctx ora_ctx;
+
+/* Creates a context from the raw mesh (INTMA, COORD, BSIDO) and the radial
+ * quadrature (eta, w); status 0 on success, negative on a reference STOP
+ * condition.  f is seeded with the INICON2 Maxwellian. */
+int ora_create(int nelem, int npoin, int nboun, const int *intma,
+               const double *coord, const int *bsido, int rorder, int torder,
+               const double *eta, const double *w, const ora_params *prm,
+               ora_ctx **out);
+void ora_destroy(ora_ctx *c);
+
+int ora_nside(const ora_ctx *c);
+int ora_nvel(const ora_ctx *c);
+const int *ora_iside(const ora_ctx *c);
+const double *ora_vcord(const ora_ctx *c);
+double ora_sumweight(const ora_ctx *c);
+
+/* f in the reference layout (3,V,E). */
+void ora_get_f(const ora_ctx *c, double *f);
+void ora_set_f(ora_ctx *c, const double *f); /* re-runs the initial MACROS */
+
+/* Per-step record. */
+typedef struct ora_step_info {
+  double sumres;   /* MAXVAL(ABS(RESIDUAL)) */
+  double logres;   /* LOG(SUMRES/SRITM1) */
+  double dte;      /* time step used */
+  double mass;     /* sum over DG nodes of DISND * RJ/6 after the step */
+  double lift, drag; /* GETFOR on the point pressure last gathered (stale) */
+} ora_step_info;
+
+int ora_step(ora_ctx *c, int nsteps, ora_step_info *info);
+
+/* DIS* node fields (3,E) and point fields (P) from the most recent MACROS. */
+void ora_get_node_fields(const ora_ctx *c, double *disnd, double *disux,
+                         double *disuy, double *disps, double *mu);
+void ora_get_point_fields(const ora_ctx *c, double *nd, double *rho,
+                          double *uvel, double *vvel, double *ps, double *temp);
+void ora_get_eta(const ora_ctx *c, double *eta);
+
+/* ---- individual per-step routines (for unit checks) ---- */
+void ora_gteqnf(int nelem, const double *disnd, const double *disux,
+                const double *disuy, const double *disps, double ux, double uy,
+                double m, double *nfo);
+void ora_getfor(int nboun, const int *bsido, const double *ps,
+                const double *coord, double *lift, double *drag);
+double ora_inflow_value(const double *cinf, double rgas, double m, double ux,
+                        double uy);
+
+/* ---- test hooks ---- */
+int ora_prepare_step(ora_ctx *c, double *dte);
+int ora_debug_ordinate(ora_ctx *c, int iv, double dte, double *nfo, double *umean,
+                       double *rhs_elc, double *rhs_edge, double *delun, double *residual);
+void ora_velocity(const ora_ctx *c, int iv, double *ux, double *uy);
+void ora_tables(const ora_ctx *c, double *nx, double *ny, double *el, double *geome,
+                double *relen, double *mmat, double *rsido, double *dtec);
+void ora_cinf(const ora_ctx *c, double *cinf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

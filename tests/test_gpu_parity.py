@@ -1,0 +1,155 @@
+"""HIP time march vs the FP64 oracle, through the C ABI.
+
+Tolerances: the FP64 path must match the oracle's f to 1e-10 relative L2
+after N steps (FP64 sums in a different order, device exp/sqrt/FMA); the
+per-step total mass to 1e-12 relative; the FP32-bulk path to 1e-5.
+"""
+import numpy as np
+import pytest
+
+from small_cases import (c1, cylinder_lobatto10, cylinder_small, lobatto10_walls, rel_l2,
+                         rel_l2_vec)
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_TOL = 1e-5
+MASS_TOL = 1e-12
+
+
+def _pair(case, precision=0, lookup=1, order=0, vchunk=0):
+    from dgbgk import _lib
+    from dgbgk.solver import Solver
+    from oracle_py import OracleRun
+    s = Solver(case, precision=precision, lookup=lookup, order=order, vchunk=vchunk)
+    o = OracleRun(case, lookup=lookup, jacobi=1 if order == _lib.ORDER_JACOBI else 0)
+    return s, o
+
+
+def _compare_steps(rs, ro, tol, mass_tol):
+    assert len(rs) == len(ro)
+    for a, b in zip(rs, ro):
+        assert abs(a["dte"] - b["dte"]) <= tol * abs(b["dte"])
+        assert abs(a["sumres"] - b["sumres"]) <= 50 * tol * abs(b["sumres"])
+        assert abs(a["mass"] - b["mass"]) <= mass_tol * abs(b["mass"])
+
+
+def test_initial_state_matches_inicon2_and_macros():
+    s, o = _pair(c1())
+    print("case c1 initial", flush=True)
+    assert rel_l2(s.get_f(), o.get_f()) < 1e-14
+    nf, of = s.node_fields(), o.node_fields()
+    for k in ("disnd", "disps"):
+        assert rel_l2(nf[k], of[k]) < 1e-13, k
+    assert rel_l2_vec(nf["disux"], nf["disuy"], of["disux"], of["disuy"]) < 1e-13
+
+
+@pytest.mark.parametrize("nsteps", [1, 20])
+def test_c1_fp64_reference_order(nsteps):
+    s, o = _pair(c1())
+    print(f"case c1 fp64 steps={nsteps}", flush=True)
+    rs, ro = s.step(nsteps), o.step(nsteps)
+    _compare_steps(rs, ro, FP64_TOL, MASS_TOL)
+    assert rel_l2(s.get_f(), o.get_f()) < FP64_TOL
+    nf, of = s.node_fields(), o.node_fields()
+    for k in ("disnd", "disps", "mu"):
+        assert rel_l2(nf[k], of[k]) < FP64_TOL, k
+    assert rel_l2_vec(nf["disux"], nf["disuy"], of["disux"], of["disuy"]) < FP64_TOL
+    pf, op = s.point_fields(), o.point_fields()
+    for k in ("nd", "rho", "ps", "temp"):
+        assert rel_l2(pf[k], op[k]) < FP64_TOL, k
+    assert rel_l2_vec(pf["uvel"], pf["vvel"], op["uvel"], op["vvel"]) < FP64_TOL
+
+
+def test_cylinder_diffuse_wall_fp64():
+    case = cylinder_small()
+    s, o = _pair(case)
+    print(f"case cylinder E={case.mesh.nelem} fp64", flush=True)
+    rs, ro = s.step(8), o.step(8)
+    _compare_steps(rs, ro, FP64_TOL, MASS_TOL)
+    assert rel_l2(s.get_f(), o.get_f()) < FP64_TOL
+    assert rel_l2(s.wall_eta(), o.eta()) < FP64_TOL
+    for a, b in zip(rs, ro):  # stale-pressure forces of GETFOR; scale p * perimeter ~ 1
+        assert abs(a["drag"] - b["drag"]) <= 1e-10 * max(abs(b["drag"]), 1.0)
+        assert abs(a["lift"] - b["lift"]) <= 1e-10 * max(abs(b["lift"]), 1.0)
+
+
+def test_cylinder_jacobi_matches_oracle_jacobi():
+    from dgbgk import _lib
+    case = cylinder_small()
+    s, o = _pair(case, order=_lib.ORDER_JACOBI)
+    print("case cylinder jacobi", flush=True)
+    rs, ro = s.step(6), o.step(6)
+    _compare_steps(rs, ro, FP64_TOL, MASS_TOL)
+    assert rel_l2(s.get_f(), o.get_f()) < FP64_TOL
+
+
+def test_reference_ring_lookup_lobatto10():
+    from dgbgk import _lib
+    case = lobatto10_walls()
+    s, o = _pair(case, lookup=_lib.LOOKUP_REFERENCE)
+    print("case lobatto10 reference lookup", flush=True)
+    rs, ro = s.step(10), o.step(10)
+    _compare_steps(rs, ro, FP64_TOL, MASS_TOL)
+    assert rel_l2(s.get_f(), o.get_f()) < FP64_TOL
+
+
+def test_reference_lookup_cylinder_lobatto10():
+    from dgbgk import _lib
+    case = cylinder_lobatto10()
+    s, o = _pair(case, lookup=_lib.LOOKUP_REFERENCE)
+    print("case cylinder lobatto10 reference lookup", flush=True)
+    rs, ro = s.step(5), o.step(5)
+    _compare_steps(rs, ro, FP64_TOL, MASS_TOL)
+    assert rel_l2(s.get_f(), o.get_f()) < FP64_TOL
+
+
+@pytest.mark.parametrize("vchunk", [1, 3, 128])
+def test_vchunk_invariance(vchunk):
+    case = cylinder_small()
+    s, o = _pair(case, vchunk=vchunk)
+    print(f"case cylinder vchunk={vchunk}", flush=True)
+    s.step(3)
+    o.step(3)
+    assert rel_l2(s.get_f(), o.get_f()) < FP64_TOL
+
+
+def test_fp32_bulk_c1():
+    from dgbgk import _lib
+    s, o = _pair(c1(), precision=_lib.DGBGK_FP32)
+    print("case c1 fp32", flush=True)
+    rs, ro = s.step(20), o.step(20)
+    assert rel_l2(s.get_f(), o.get_f()) < FP32_TOL
+    nf, of = s.node_fields(), o.node_fields()
+    for k in ("disnd", "disps"):
+        assert rel_l2(nf[k], of[k]) < FP32_TOL, k
+    assert rel_l2_vec(nf["disux"], nf["disuy"], of["disux"], of["disuy"]) < FP32_TOL
+    for a, b in zip(rs, ro):
+        assert abs(a["mass"] - b["mass"]) <= 1e-6 * abs(b["mass"])
+
+
+def test_fp32_bulk_cylinder():
+    from dgbgk import _lib
+    case = cylinder_small()
+    s, o = _pair(case, precision=_lib.DGBGK_FP32)
+    print("case cylinder fp32", flush=True)
+    s.step(8)
+    o.step(8)
+    assert rel_l2(s.get_f(), o.get_f()) < FP32_TOL
+
+
+def test_set_get_roundtrip_and_seed():
+    from dgbgk.solver import Solver
+    case = c1(n=8)
+    s = Solver(case)
+    rng = np.random.default_rng(3)
+    f = rng.random((case.mesh.nelem, case.nvel, 3))
+    s.set_f(f)
+    assert np.array_equal(s.get_f(), f)
+    # seeding a uniform freestream reproduces INICON2's f
+    P = case.mesh.npoin
+    pr = case.prm
+    rho = np.full(P, pr["p1"] * 1e5 / (pr["rgas"] * pr["t1"]))
+    s.seed(rho, np.full(P, pr["u0"]), np.full(P, pr["v0"]), np.full(P, pr["t1"]))
+    s2 = Solver(case)
+    assert rel_l2(s.get_f(), s2.get_f()) < 1e-14
