@@ -52,6 +52,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--elems", type=int, default=100_000)
+    ap.add_argument("--vchunk", type=int, default=0, help="ordinates per update block (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     args = ap.parse_args()
@@ -66,7 +67,7 @@ def main():
     t0 = time.time()
     case = cases.config2(target_elems=args.elems)
     prec = _lib.DGBGK_FP64 if args.precision == "fp64" else _lib.DGBGK_FP32
-    s = Solver(case, precision=prec)
+    s = Solver(case, precision=prec, vchunk=args.vchunk)
     setup_s = time.time() - t0
     E, V = case.mesh.nelem, case.nvel
     dofv = 3 * E * V

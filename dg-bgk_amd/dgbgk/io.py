@@ -72,7 +72,7 @@ def write_lobatto(path: str, eta, w):
     with open(path, "w") as fh:
         fh.write(f"{len(eta)}\n")
         for a, b in zip(eta, w):
-            fh.write(f"{a!r}\t{b!r}\n")
+            fh.write(f"{float(a)!r}\t{float(b)!r}\n")
 
 
 def read_plt(path: str) -> Mesh:
@@ -126,7 +126,7 @@ def write_plt(path: str, mesh: Mesh, title: str = "title"):
             fh.write(f"{i + 1:10d}{t[0]:10d}{t[1]:10d}{t[2]:10d}         0\n")
         fh.write("  coordinates \n")
         for i, c in enumerate(mesh.coord):
-            fh.write(f"{i + 1:10d} {c[0]!r} {c[1]!r}\n")
+            fh.write(f"{i + 1:10d} {float(c[0])!r} {float(c[1])!r}\n")
         fh.write("  boundary sides \n")
         for b in mesh.bsido:
             fh.write(" ".join(f"{int(x):10d}" for x in b) + "\n")
@@ -165,3 +165,26 @@ def results_block(itime: int, a, b, c) -> str:
     for ip in range(len(a)):
         out.append(f"{ip + 1:5d}    {fortran_e(a[ip])} {fortran_e(b[ip])} {fortran_e(c[ip])} ")
     return "\n".join(out) + "\n"
+
+
+def write_run_dir(case, directory: str, ntime: int, forceout: int = 1,
+                  deck_name: str = "run.con") -> str:
+    """Writes deck + mesh + velocity file of a Case as a reference-style run
+    directory; returns the deck path."""
+    os.makedirs(directory, exist_ok=True)
+    write_plt(os.path.join(directory, "mesh.plt"), case.mesh)
+    write_lobatto(os.path.join(directory, "velocity.txt"), case.eta, case.w)
+    p = case.prm
+    keys = {
+        "TORDER": int(case.torder), "rv": p["rv"], "FORCEOUT": int(forceout),
+        "NTIME": int(ntime), "IMMAT": 1, "ALPHA": p["alpha"], "CSAFM": p["csafm"],
+        "RS": 0, "T1": p["t1"], "P1": p["p1"], "U0": p["u0"], "V0": p["v0"],
+        "INF": int(p.get("inf", 1)), "CINF": list(p["cinf"]), "NVSPACEPART": 1,
+        "R": p["rgas"], "d": p["d"], "M": p["m"],
+        "LobattoFile": "velocity.txt", "PSpaceFile": "mesh.plt", "OutFile": "FILE.OUT",
+        "ResidualFile": "RESIDUAL.RES", "ResultsFile1": "RESULTS1.RES",
+        "ResultsFile2": "RESULTS2.RES",
+    }
+    path = os.path.join(directory, deck_name)
+    write_deck(path, keys)
+    return path

@@ -1,0 +1,65 @@
+"""The Fortran 90 host (dgbgk_run) end to end: deck + mesh + velocity file
+in the reference's formats, RESIDUAL.RES and RESULTS*.RES out, checked
+against the oracle run on the same deck."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from small_cases import cylinder_small
+
+HOST = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                    "dg-bgk_amd", "dgbgk", "dgbgk_run")
+
+
+def test_fortran_host_check_mode(tmp_path):
+    """Deck/mesh/velocity parsing without a GPU: sizes match the library's
+    host setup."""
+    from dgbgk import io
+    from dgbgk.solver import Setup
+    if not os.path.exists(HOST):
+        pytest.skip("Fortran host not built")
+    case = cylinder_small(elems=800)
+    deck = io.write_run_dir(case, str(tmp_path), ntime=5)
+    out = subprocess.run([HOST, deck, "--check"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    line = [x for x in out.stdout.splitlines() if x.startswith("CHECK")][0].split()
+    s = Setup(case)
+    assert [int(x) for x in line[1:6]] == [case.mesh.nelem, case.mesh.npoin,
+                                          case.mesh.nboun, s.S, s.V]
+    fo = open(os.path.join(tmp_path, "FILE.OUT")).read().split()
+    assert str(case.mesh.nelem) in fo and str(case.nvel) in fo
+
+
+@pytest.mark.gpu
+def test_fortran_host_matches_oracle(tmp_path):
+    from dgbgk import io
+    from oracle_py import OracleRun
+    assert os.path.exists(HOST), "Fortran host not built"
+    case = cylinder_small(elems=1200)
+    nt = 12
+    deck = io.write_run_dir(case, str(tmp_path), ntime=nt, forceout=1)
+    print(f"fortran host cylinder E={case.mesh.nelem} steps={nt}", flush=True)
+    out = subprocess.run([HOST, deck], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    rows = [list(map(float, l.replace("D", "E").split()))
+            for l in open(os.path.join(tmp_path, "RESIDUAL.RES")) if l.strip()]
+    assert len(rows) == nt
+    o = OracleRun(case)
+    ro = o.step(nt)
+    for row, r in zip(rows, ro):
+        assert abs(row[1] - r["logres"]) <= 1e-8 * max(1.0, abs(r["logres"]))
+        assert abs(row[3] - r["drag"]) <= 1e-9 * max(1.0, abs(r["drag"]))
+    # RESULTS files: step-0 block, final block (NTIME+1) with the fields of
+    # the last gather (step 0 here, as NTIME < 1000)
+    txt = open(os.path.join(tmp_path, "RESULTS1.RES")).read().splitlines()
+    heads = [i for i, l in enumerate(txt) if l.startswith("RESULTS")]
+    assert txt[heads[0]] == "RESULTS        1       0    2      1      0"
+    assert txt[heads[-1]] == f"RESULTS        1   {nt + 1:5d}    2      1      0"
+    o0 = OracleRun(case)
+    pf = o0.point_fields()
+    first = np.array([list(map(float, l.split()[1:4])) for l in txt[heads[0] + 1:heads[1]]])
+    assert np.allclose(first[:, 0], pf["nd"], rtol=2e-5)
