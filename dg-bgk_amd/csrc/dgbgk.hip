@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -78,11 +79,18 @@ __device__ __forceinline__ double block_sum(double x, double* sh) {
   return x;
 }
 
-// Neumaier compensated accumulation
+// Kahan compensated accumulation (FP32-bulk moments); the FP64 path sums
+// plainly in ordinate order, as the reference does.
+template <bool kComp>
 __device__ __forceinline__ void nsum(double& s, double& c, double x) {
-  const double t = s + x;
-  c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
-  s = t;
+  if (kComp) {
+    const double y = x - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  } else {
+    s += x;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -144,17 +152,34 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
   const T dte = (T)a.scal[0];
   const T c13 = T(1) / T(3);
   const T alpha = a.alpha;
-  double res = 0.0;
+  T res = 0;
 
+  // software pipeline: the next ordinate's f and velocity are loaded while
+  // the current one is computed, so the loads' latency is hidden behind the
+  // barrier-separated phases
+  T fq[3] = {};
+  T uxq = a.vtab[v0], uyq = a.vtab[V + v0];
+  if (active) {
+    const T* fp = a.fcur + (size_t)v0 * N3 + 3 * (size_t)e;
+    fq[0] = fp[0];
+    fq[1] = fp[1];
+    fq[2] = fp[2];
+  }
   for (int v = v0; v < v1; ++v) {
     const int buf = (v - v0) & 1;
-    const T ux = a.vtab[v], uy = a.vtab[V + v];
-    T f[3] = {}, coll[3] = {}, U = 0;
+    const T ux = uxq, uy = uyq;
+    T f[3] = {fq[0], fq[1], fq[2]}, coll[3] = {}, U = 0;
+    if (v + 1 < v1) {
+      uxq = a.vtab[v + 1];
+      uyq = a.vtab[V + v + 1];
+      if (active) {
+        const T* fp = a.fcur + (size_t)(v + 1) * N3 + 3 * (size_t)e;
+        fq[0] = fp[0];
+        fq[1] = fp[1];
+        fq[2] = fp[2];
+      }
+    }
     if (active) {
-      const T* fp = a.fcur + (size_t)v * N3 + 3 * (size_t)e;
-      f[0] = fp[0];
-      f[1] = fp[1];
-      f[2] = fp[2];
       T dive = 0, cs = 0;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -179,20 +204,23 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
     T rhs[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) rhs[k] = (gx[k] * area) * fxme + (gy[k] * area) * fyme;
-    // edges (src/edgflxopt.f90:72-125, src/getbou.f90:142-281)
+    // edges (src/edgflxopt.f90:72-125, src/getbou.f90:142-281).  Local edge k
+    // runs from node k to node k1 = (k+1)%3; the side's points p1, p2 are
+    // (k, k1) when this element is the side's left element and (k1, k)
+    // when it is the right one, so every register index is compile-time.
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
+      const int k1 = (k + 1) % 3;
       const int in = info[k];
       const int kind = in & 3;
       if (kind == 1) continue;  // inflow side: no flux
-      const int a1 = (in >> 2) & 3, a2 = (in >> 4) & 3;
       T g1, g2, r1, r2;
       if (kind == 0) {
         const bool left = (in >> 10) & 1;
         const T scpr = ux * enx[k] + uy * eny[k];
         if ((scpr > T(0)) == left) {
-          g1 = f[a1] - U;
-          g2 = f[a2] - U;
+          g1 = (left ? f[k] : f[k1]) - U;
+          g2 = (left ? f[k1] : f[k]) - U;
         } else {
           const int ns = nsl[k], b1 = (in >> 6) & 3, b2 = (in >> 8) & 3;
           const T un = s_u[buf][ns];
@@ -200,25 +228,26 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
           g2 = s_f[buf][3 * ns + b2] - un;
         }
         edge_rhsi(ux, uy, enx[k], eny[k], eal[k], g1, g2, r1, r2);
-        if (left) { rhs[a1] += r1; rhs[a2] += r2; }
-        else { rhs[a1] -= r1; rhs[a2] -= r2; }
+        if (left) { rhs[k] += r1; rhs[k1] += r2; }
+        else { rhs[k1] -= r1; rhs[k] -= r2; }
         continue;
       }
+      // boundary side: this element is its left element (p1, p2) = (k, k1)
       int i1 = -1, ws = -1;
       if (kind == 3) {
         ws = nsl[k];
         i1 = a.refl_i[2 * ((size_t)ws * V + v)];
       }
       if (i1 < 0) {  // outflow, or molecules moving into the wall
-        g1 = f[a1] - U;
-        g2 = f[a2] - U;
+        g1 = f[k] - U;
+        g2 = f[k1] - U;
       } else if (!kGS) {  // reflected distribution, previous-step values
         const int i2 = a.refl_i[2 * ((size_t)ws * V + v) + 1];
         const T* rt = a.refl_t + 3 * ((size_t)ws * V + v);
         const T zd1 = rt[0], zd2 = rt[1], mw = rt[2];
         const T* fb = a.fcur + 3 * (size_t)e;
-        const T fa1 = fb[(size_t)i1 * N3 + a1], fb1 = fb[(size_t)i2 * N3 + a1];
-        const T fa2 = fb[(size_t)i1 * N3 + a2], fb2 = fb[(size_t)i2 * N3 + a2];
+        const T fa1 = fb[(size_t)i1 * N3 + k], fb1 = fb[(size_t)i2 * N3 + k];
+        const T fa2 = fb[(size_t)i1 * N3 + k1], fb2 = fb[(size_t)i2 * N3 + k1];
         const T rf1 = (zd2 * fa1 + zd1 * fb1) / (zd1 + zd2);
         const T rf2 = (zd2 * fa2 + zd1 * fb2) / (zd1 + zd2);
         const T mf = mw * (T)a.eta[a.wall_bside[ws]];
@@ -228,8 +257,8 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
         continue;  // the ordered wall pass adds it
       }
       edge_rhsi(ux, uy, enx[k], eny[k], eal[k], g1, g2, r1, r2);
-      rhs[a1] += r1;
-      rhs[a2] += r2;
+      rhs[k] += r1;
+      rhs[k1] += r2;
     }
     if (kGS && wsl >= 0) {
       T* wp = a.wpart + ((size_t)wsl * V + v) * 6;
@@ -244,14 +273,14 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const T d = mmat * (dte * rhs[k]) + coll[k] * dte;
-      res = fmax(res, (double)fabs(d));
+      res = fmax(res, fabs(d));
       T fn = f[k] + d;
       fn = fn < T(0) ? T(0) : fn;
       if ((mask >> k) & 1) fn = fin;
       fo[k] = fn;
     }
   }
-  const double r = block_max(res, s_red);
+  const double r = block_max((double)res, s_red);
   if (threadIdx.x == 0) a.res_part[logical] = r;
 }
 
@@ -266,6 +295,11 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
 template <typename T>
 __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
   __shared__ double s_red[8];
+  // per thread, the ring's freshly advanced f (when reflections stay in the
+  // ring): the in-place reads of ordinates < v come from LDS
+  extern __shared__ unsigned char s_dyn[];
+  T* s_ring = reinterpret_cast<T*>(s_dyn) + (size_t)threadIdx.x * 3 * a.T_;
+  const bool use_lds = a.ngroups > 1;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   double res = 0.0;
   if (gid < a.nwe * a.ngroups) {
@@ -302,10 +336,17 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
         const int i1 = a.refl_i[2 * rk];
         if (i1 < 0) continue;
         const int i2 = a.refl_i[2 * rk + 1];
-        const int a1 = (info[k] >> 2) & 3, a2 = (info[k] >> 4) & 3;
+        const int a1 = k, a2 = (k + 1) % 3;  // boundary side: (p1, p2) = (k, k+1)
         const T zd1 = a.refl_t[3 * rk], zd2 = a.refl_t[3 * rk + 1], mw = a.refl_t[3 * rk + 2];
-        const T* s1 = (i1 < v ? a.fnxt : a.fcur) + (size_t)i1 * N3 + 3 * (size_t)e;
-        const T* s2 = (i2 < v ? a.fnxt : a.fcur) + (size_t)i2 * N3 + 3 * (size_t)e;
+        const T* s1;
+        const T* s2;
+        if (use_lds) {
+          s1 = i1 < v ? s_ring + 3 * (i1 - vb) : a.fcur + (size_t)i1 * N3 + 3 * (size_t)e;
+          s2 = i2 < v ? s_ring + 3 * (i2 - vb) : a.fcur + (size_t)i2 * N3 + 3 * (size_t)e;
+        } else {
+          s1 = (i1 < v ? a.fnxt : a.fcur) + (size_t)i1 * N3 + 3 * (size_t)e;
+          s2 = (i2 < v ? a.fnxt : a.fcur) + (size_t)i2 * N3 + 3 * (size_t)e;
+        }
         const T rf1 = (zd2 * s1[a1] + zd1 * s2[a1]) / (zd1 + zd2);
         const T rf2 = (zd2 * s1[a2] + zd1 * s2[a2]) / (zd1 + zd2);
         const T mf = mw * (T)a.eta[a.wall_bside[ws]];
@@ -327,6 +368,7 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
         fn = fn < T(0) ? T(0) : fn;
         if ((mask >> k) & 1) fn = fin;
         fo[k] = fn;
+        if (use_lds) s_ring[3 * (v - vb) + k] = fn;
       }
     }
   }
@@ -342,6 +384,7 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
+  constexpr bool kComp = sizeof(T) < sizeof(double);
   __shared__ double s_red[8];
   const size_t N3 = 3 * (size_t)a.E;
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -363,23 +406,23 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const double fw = fv[j] * wv[v + j];
-        nsum(s0, c0, fw);
-        nsum(s1, c1, cxb[v + j] * fw);
-        nsum(s2, c2, cyb[v + j] * fw);
+        nsum<kComp>(s0, c0, fw);
+        nsum<kComp>(s1, c1, cxb[v + j] * fw);
+        nsum<kComp>(s2, c2, cyb[v + j] * fw);
       }
     }
     for (; v < V; ++v) {
       const double fw = (double)fp[(size_t)v * N3] * wv[v];
-      nsum(s0, c0, fw);
-      nsum(s1, c1, cxb[v] * fw);
-      nsum(s2, c2, cyb[v] * fw);
+      nsum<kComp>(s0, c0, fw);
+      nsum<kComp>(s1, c1, cxb[v] * fw);
+      nsum<kComp>(s2, c2, cyb[v] * fw);
     }
-    const double n = s0 + c0;
+    const double n = s0 - c0;
     double nd = 0.0, ux = 0.0, uy = 0.0;
     if (n >= 1e-20) {
       nd = n;
-      ux = (s1 + c1) / n;
-      uy = (s2 + c2) / n;
+      ux = (s1 - c1) / n;
+      uy = (s2 - c2) / n;
     }
     double sp = 0, cp = 0, st = 0, ct = 0;
     const bool dens = n >= 1e-20;
@@ -392,24 +435,24 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
       for (int j = 0; j < 4; ++j) {
         const double fw = fv[j] * wv[v + j];
         const double dx = cxb[v + j] - ux, dy = cyb[v + j] - uy;
-        nsum(sp, cp, 0.5 * (dx * dx + dy * dy) * fw);
+        nsum<kComp>(sp, cp, 0.5 * (dx * dx + dy * dy) * fw);
         if (dens) {
           const double ex = ux - cxa[v + j], ey = uy - cya[v + j];
-          nsum(st, ct, sqrt(ex * ex + ey * ey) * fw);
+          nsum<kComp>(st, ct, sqrt(ex * ex + ey * ey) * fw);
         }
       }
     }
     for (; v < V; ++v) {
       const double fw = (double)fp[(size_t)v * N3] * wv[v];
       const double dx = cxb[v] - ux, dy = cyb[v] - uy;
-      nsum(sp, cp, 0.5 * (dx * dx + dy * dy) * fw);
+      nsum<kComp>(sp, cp, 0.5 * (dx * dx + dy * dy) * fw);
       if (dens) {
         const double ex = ux - cxa[v], ey = uy - cya[v];
-        nsum(st, ct, sqrt(ex * ex + ey * ey) * fw);
+        nsum<kComp>(st, ct, sqrt(ex * ex + ey * ey) * fw);
       }
     }
-    const double ps = (sp + cp) * a.mm;
-    const double ctv = dens ? (st + ct) / n : 0.0;
+    const double ps = (sp - cp) * a.mm;
+    const double ctv = dens ? (st - ct) / n : 0.0;
     mu = sqrt(2.0) * ctv * a.sigt * nd;
     a.mom[k] = nd;
     a.mom[N3 + k] = ux;
@@ -468,7 +511,9 @@ __global__ void __launch_bounds__(256) k_points(const double* __restrict__ mom,
                                                 double* __restrict__ pts,
                                                 double* __restrict__ ps_glob, int E,
                                                 int P, double mm, double rgas,
-                                                int gather) {
+                                                int gather, const double* __restrict__ scal) {
+  // in a step, gather only on the reference's GETMAC steps (every 1000)
+  if (scal != nullptr && (((long long)scal[6] + 1) % 1000) != 0) return;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= P) return;
   const size_t N3 = 3 * (size_t)E;
@@ -514,7 +559,7 @@ __global__ void __launch_bounds__(256) k_final(const double* __restrict__ res_pa
                                                const double* __restrict__ coord,
                                                const double* __restrict__ ps_glob,
                                                double* __restrict__ scal,
-                                               StepRec* __restrict__ rec, int itime,
+                                               StepRec* __restrict__ rec, int rec_cap,
                                                int record) {
   __shared__ double s_red[8];
   double r = 0.0, m = 0.0, ms = 0.0, lift = 0.0, drag = 0.0;
@@ -541,13 +586,17 @@ __global__ void __launch_bounds__(256) k_final(const double* __restrict__ res_pa
   drag = block_sum(drag, s_red);
   if (threadIdx.x == 0) {
     if (record >= 0) {
+      // scal[6] counts completed steps (ITIME of src/advnce.f90:220)
+      const long long itime = (long long)scal[6] + 1;
+      scal[6] = (double)itime;
+      const int slot = (int)((itime - 1) % rec_cap);
       if (itime == 1) scal[1] = r;
-      rec[record].sumres = r;
-      rec[record].logres = log(r / scal[1]);
-      rec[record].dte = scal[0];
-      rec[record].mass = ms;
-      rec[record].lift = lift;
-      rec[record].drag = drag;
+      rec[slot].sumres = r;
+      rec[slot].logres = log(r / scal[1]);
+      rec[slot].dte = scal[0];
+      rec[slot].mass = ms;
+      rec[slot].lift = lift;
+      rec[slot].drag = drag;
     }
     scal[2] = lift;
     scal[3] = drag;
@@ -656,9 +705,12 @@ struct Ctx {
          *mu_part = nullptr, *mass_part = nullptr, *scal = nullptr, *ps4 = nullptr;
   StepRec* rec = nullptr;
   int rec_cap = 0;
+  int wall_tpb = 256;
   int ntiles = 0, vchunk = 1, nvchunks = 1, nblk_upd = 0, nblk_wall = 0, ngroups = 1,
       nblk_mom = 0, neta = 0, nforce = 0;
   int itime = 0;
+  bool use_graph = true;
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // by current f buffer
   // timing: events recorded around launches, read after a synchronisation
   bool timing = false;
   struct Ev {
@@ -695,16 +747,17 @@ int launch_moments(Ctx& c) {
 int launch_final(Ctx& c, int record) {
   hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.res_part,
                      c.nblk_upd + c.nblk_wall, c.mu_part, c.mass_part, c.nblk_mom, c.fside,
-                     c.nforce, c.coord, c.ps_glob, c.scal, c.rec, c.itime, record);
+                     c.nforce, c.coord, c.ps_glob, c.scal, c.rec, c.rec_cap, record);
   HIPCK(hipGetLastError());
   return kOk;
 }
 
-int launch_points(Ctx& c, int gather) {
+// in_step: gate on the device step counter (GETMAC every 1000 steps)
+int launch_points(Ctx& c, int gather, bool in_step = false) {
   const int P = c.p.P;
   hipLaunchKernelGGL(k_points, dim3((P + 255) / 256), dim3(256), 0, c.stream, c.mom, c.rj,
                      c.pt_ptr, c.pt_node, c.pts, c.ps_glob, c.p.E, P, c.p.prm.m / kNa,
-                     c.p.prm.rgas, gather);
+                     c.p.prm.rgas, gather, in_step ? (const double*)c.scal : nullptr);
   HIPCK(hipGetLastError());
   return kOk;
 }
@@ -741,9 +794,8 @@ void tend(Ctx& c, int k) {
 }
 
 template <typename T>
-int step_once(Ctx& c, int record) {
+int step_once(Ctx& c) {
   const Prep& p = c.p;
-  c.itime += 1;
   const T* fc = (const T*)c.f[c.cur];
   T* fn = (T*)c.f[1 - c.cur];
   if (c.neta) {
@@ -778,7 +830,8 @@ int step_once(Ctx& c, int record) {
     w.res_part = c.res_part + c.nblk_upd; w.alpha = (T)p.prm.alpha; w.E = p.E; w.V = p.V;
     w.T_ = p.T; w.ngroups = c.ngroups; w.nwe = (int)p.wall_elems.size();
     tbegin(c, 1);
-    hipLaunchKernelGGL(k_wall<T>, dim3(c.nblk_wall), dim3(256), 0, c.stream, w);
+    const size_t lds = c.ngroups > 1 ? (size_t)c.wall_tpb * 3 * p.T * sizeof(T) : 0;
+    hipLaunchKernelGGL(k_wall<T>, dim3(c.nblk_wall), dim3(c.wall_tpb), lds, c.stream, w);
     HIPCK(hipGetLastError());
     tend(c, 1);
   }
@@ -787,16 +840,59 @@ int step_once(Ctx& c, int record) {
   int st = launch_moments<T>(c);
   if (st) return st;
   tend(c, 2);
-  if (c.itime % 1000 == 0) {
-    tbegin(c, 5);
-    st = launch_points(c, 1);
-    if (st) return st;
-    tend(c, 5);
-  }
+  tbegin(c, 5);
+  st = launch_points(c, 1, true);
+  if (st) return st;
+  tend(c, 5);
   tbegin(c, 4);
-  st = launch_final(c, record);
+  st = launch_final(c, 0);
   tend(c, 4);
   return st;
+}
+
+// A captured sequence of kGraphSteps steps (an even count, so the two f
+// buffers end where they started) replayed as one HIP graph.
+constexpr int kGraphSteps = 64;
+
+template <typename T>
+int capture_graph(Ctx& c) {
+  const int slot = c.cur;
+  if (c.graph_exec[slot]) return kOk;
+  HIPCK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  int st = kOk;
+  for (int s = 0; s < kGraphSteps && st == kOk; ++s) st = step_once<T>(c);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c.stream, &g);
+  if (st) {
+    if (g) hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return fail(kErrHip, std::string("graph capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&c.graph_exec[slot], g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c.graph_exec[slot] = nullptr;
+    return fail(kErrHip, std::string("graph instantiate: ") + hipGetErrorString(e));
+  }
+  return kOk;
+}
+
+template <typename T>
+int run_steps(Ctx& c, int n) {
+  int done = 0;
+  if (c.use_graph && !c.timing && n >= kGraphSteps) {
+    int st = capture_graph<T>(c);
+    if (st) return st;
+    while (n - done >= kGraphSteps) {
+      HIPCK(hipGraphLaunch(c.graph_exec[c.cur], c.stream));
+      done += kGraphSteps;
+    }
+  }
+  for (; done < n; ++done) {
+    int st = step_once<T>(c);
+    if (st) return st;
+  }
+  return kOk;
 }
 
 template <typename T>
@@ -1033,6 +1129,7 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   }
   c.prec = prm->precision;
   c.gs = prm->order == DGBGK_ORDER_REFERENCE ? 1 : 0;
+  if (const char* ng = std::getenv("DGBGK_NO_GRAPH")) c.use_graph = ng[0] == '0';
   c.tsz = c.prec ? 4 : 8;
   const Prep& p = c.p;
   auto cleanup = [&](int code) {
@@ -1059,7 +1156,14 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   c.nblk_upd = c.ntiles * c.nvchunks;
   c.ngroups = p.ring_separable ? p.R : 1;
   const int nwe = (int)p.wall_elems.size();
-  c.nblk_wall = (c.gs && nwe) ? (nwe * c.ngroups + 255) / 256 : 0;
+  // k_wall: threads per block so each thread's ring fits in <= 64 KB of LDS
+  c.wall_tpb = 256;
+  if (c.ngroups > 1) {
+    const size_t per = 3 * (size_t)p.T * c.tsz;
+    while (c.wall_tpb > 64 && (size_t)c.wall_tpb * per > 65536) c.wall_tpb >>= 1;
+    if ((size_t)c.wall_tpb * per > 65536) c.ngroups = 1;  // ring too long: no LDS staging
+  }
+  c.nblk_wall = (c.gs && nwe) ? (nwe * c.ngroups + c.wall_tpb - 1) / c.wall_tpb : 0;
   c.nblk_mom = (int)((3 * (size_t)p.E + 255) / 256);
   c.neta = (int)p.eta_side.size();
   c.nforce = (int)p.force_p1.size();
@@ -1105,6 +1209,8 @@ void dgbgk_destroy(dgbgk_ctx* h) {
     hipEventDestroy(e.a);
     hipEventDestroy(e.b);
   }
+  for (auto& g : c.graph_exec)
+    if (g) hipGraphExecDestroy(g);
   for (void* q : c.owned) hipFree(q);
   if (c.stream) hipStreamDestroy(c.stream);
   delete h;
@@ -1116,13 +1222,17 @@ int dgbgk_step(dgbgk_ctx* h, int nsteps, dgbgk_step_info* info) {
   int done = 0;
   while (done < nsteps) {
     const int batch = std::min(nsteps - done, c.rec_cap);
-    for (int s = 0; s < batch; ++s) {
-      int st = c.prec ? step_once<float>(c, s) : step_once<double>(c, s);
-      if (st) return st;
-    }
+    const int first = c.itime % c.rec_cap;  // slot of this batch's first record
+    int st = c.prec ? run_steps<float>(c, batch) : run_steps<double>(c, batch);
+    if (st) return st;
+    c.itime += batch;
     if (info) {
-      HIPCK(hipMemcpyAsync(info + done, c.rec, batch * sizeof(StepRec), hipMemcpyDeviceToHost,
-                           c.stream));
+      const int n1 = std::min(batch, c.rec_cap - first);
+      HIPCK(hipMemcpyAsync(info + done, c.rec + first, n1 * sizeof(StepRec),
+                           hipMemcpyDeviceToHost, c.stream));
+      if (batch > n1)
+        HIPCK(hipMemcpyAsync(info + done + n1, c.rec, (batch - n1) * sizeof(StepRec),
+                             hipMemcpyDeviceToHost, c.stream));
     }
     HIPCK(hipStreamSynchronize(c.stream));
     tharvest(c);

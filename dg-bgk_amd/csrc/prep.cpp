@@ -343,6 +343,16 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
   }
   for (size_t k = 0; k < seen.size(); ++k)
     if (!seen[k]) { msg = "element edge without a side"; return kErrMesh; }
+  // the update kernel relies on: local edge k runs from node k to node
+  // (k+1)%3, and the side's points (p1, p2) are those nodes in that order
+  // for the left element and reversed for the right element
+  for (int e = 0; e < E; ++e)
+    for (int k = 0; k < 3; ++k) {
+      const ElemEdge& ed = p.edges[3 * (size_t)e + k];
+      const int k1 = (k + 1) % 3;
+      const bool ok = ed.sign > 0 ? (ed.a1 == k && ed.a2 == k1) : (ed.a1 == k1 && ed.a2 == k);
+      if (!ok) { msg = "inconsistent element edge orientation"; return kErrMesh; }
+    }
   // inflow node mask: every DG node on a point of a type-1 side
   {
     std::vector<uint8_t> ptin(npoin + 1, 0);

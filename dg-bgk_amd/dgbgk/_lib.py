@@ -31,7 +31,9 @@ EXPORTS = [
     "dgbgk_forces", "dgbgk_wall_eta", "dgbgk_sizes", "dgbgk_setup_tables",
     "dgbgk_setup_create", "dgbgk_setup_sizes", "dgbgk_setup_tables_host",
     "dgbgk_setup_destroy", "dgbgk_timing", "dgbgk_kernel_time", "dgbgk_sync",
-    "dgbgk_dof_bytes",
+    "dgbgk_dof_bytes", "dgbgk_mlp_create", "dgbgk_mlp_destroy", "dgbgk_mlp_last_error",
+    "dgbgk_mlp_nparams", "dgbgk_mlp_set_params", "dgbgk_mlp_get_params", "dgbgk_mlp_forward",
+    "dgbgk_mlp_loss_grad", "dgbgk_mlp_set_data", "dgbgk_mlp_train",
 ]
 
 
@@ -107,6 +109,19 @@ def lib():
     L.dgbgk_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
     L.dgbgk_sync.argtypes = [vp]
     L.dgbgk_dof_bytes.argtypes = [vp, C.POINTER(C.c_int)]
+    fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    ofp = _opt(fp)
+    L.dgbgk_mlp_create.argtypes = [C.c_int] * 5 + [C.POINTER(vp)]
+    L.dgbgk_mlp_destroy.argtypes = [vp]
+    L.dgbgk_mlp_destroy.restype = None
+    L.dgbgk_mlp_last_error.restype = C.c_char_p
+    L.dgbgk_mlp_nparams.argtypes = [vp, C.POINTER(C.c_long)]
+    L.dgbgk_mlp_set_params.argtypes = [vp, fp]
+    L.dgbgk_mlp_get_params.argtypes = [vp, fp]
+    L.dgbgk_mlp_forward.argtypes = [vp, C.c_long, fp, fp]
+    L.dgbgk_mlp_loss_grad.argtypes = [vp, C.c_int, fp, fp, C.POINTER(C.c_float), ofp]
+    L.dgbgk_mlp_set_data.argtypes = [vp, C.c_long, fp, fp]
+    L.dgbgk_mlp_train.argtypes = [vp, C.c_int, C.c_float, C.c_uint, ofp]
     _L = L
     return L
 

@@ -152,6 +152,27 @@ int dgbgk_sync(dgbgk_ctx *ctx);
  * update of the update kernel (one read and one write of f). */
 int dgbgk_dof_bytes(dgbgk_ctx *ctx, int *bytes_per_value);
 
+/* ---- learned warm start: coordinate MLP (no counterpart in the reference) ----
+ * fin -> width (SiLU), (hidden-1) x width -> width (SiLU), width -> fout.
+ * BF16 weights and activations, FP32 accumulation (MFMA) and FP32 master
+ * weights, Adam.  Parameters are packed per layer as W [dout][din] then
+ * b [dout].  width and batch must be multiples of 64. */
+typedef struct dgbgk_mlp dgbgk_mlp;
+int dgbgk_mlp_create(int fin, int width, int hidden, int fout, int batch, dgbgk_mlp **out);
+void dgbgk_mlp_destroy(dgbgk_mlp *m);
+const char *dgbgk_mlp_last_error(void);
+int dgbgk_mlp_nparams(dgbgk_mlp *m, long *n);
+int dgbgk_mlp_set_params(dgbgk_mlp *m, const float *params);
+int dgbgk_mlp_get_params(dgbgk_mlp *m, float *params);
+/* y[n][fout] = net(x[n][fin]) */
+int dgbgk_mlp_forward(dgbgk_mlp *m, long n, const float *x, float *y);
+/* one batch of rows (multiple of 64, <= batch): MSE loss and its gradient */
+int dgbgk_mlp_loss_grad(dgbgk_mlp *m, int rows, const float *x, const float *y, float *loss,
+                        float *grad);
+/* device-resident training set and Adam steps on random batches */
+int dgbgk_mlp_set_data(dgbgk_mlp *m, long n, const float *x, const float *y);
+int dgbgk_mlp_train(dgbgk_mlp *m, int steps, float lr, unsigned seed, float *losses);
+
 #ifdef __cplusplus
 }
 #endif
