@@ -45,17 +45,39 @@ def _traffic_from_profiles(workload: str, precision: str):
     return None
 
 
+def copy_calibration(nbytes: int):
+    """Three plain device-to-device copies of a known size: under rocprofv3
+    --pmc, the counters' value for this copy gives the scale from counter
+    units to bytes read and written."""
+    import torch
+    a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {nbytes} bytes x 3", file=sys.stderr)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--elems", type=int, default=100_000)
+    ap.add_argument("--config", choices=["cylinder", "particle"], default="cylinder",
+                    help="cylinder: 100k triangles x 512 ordinates (headline); "
+                         "particle: 1M triangles x 2048 ordinates")
+    ap.add_argument("--elems", type=int, default=0, help="override the mesh size")
     ap.add_argument("--vchunk", type=int, default=0, help="ordinates per update block (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run three device-to-device copies of BYTES bytes (the "
+                         "reference kernel that scales rocprofv3's FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the time march has no multi-GPU path "
               "in this build (see DESIGN.md for how it would shard)", file=sys.stderr)
@@ -65,13 +87,20 @@ def main():
     from dgbgk.solver import Solver
 
     t0 = time.time()
-    case = cases.config2(target_elems=args.elems)
+    if args.config == "cylinder":
+        case = cases.cylinder_flow(target_elems=args.elems or 100_000)
+        wl = "cylinder flow, 20Dx10D box, ~100k triangles x 16x32 ordinates, Mach 0.5, Kn 0.05"
+        mach, kn = 0.5, 0.05
+    else:
+        case = cases.particle_drag(target_elems=args.elems or 1_000_000)
+        wl = "particle drag, ~1M triangles x 32x64 ordinates, Mach 0.3, Kn 1"
+        mach, kn = 0.3, 1.0
     prec = _lib.DGBGK_FP64 if args.precision == "fp64" else _lib.DGBGK_FP32
     s = Solver(case, precision=prec, vchunk=args.vchunk)
     setup_s = time.time() - t0
     E, V = case.mesh.nelem, case.nvel
     dofv = 3 * E * V
-    print(f"# workload cylinder E={E} V={V} ({case.rorder}x{case.torder}) "
+    print(f"# workload {args.config} E={E} V={V} ({case.rorder}x{case.torder}) "
           f"{args.precision} setup {setup_s:.1f}s", file=sys.stderr, flush=True)
 
     s.step(args.warmup, records=False)
@@ -94,7 +123,7 @@ def main():
     avg_upd_s = (upd_ms / max(upd_n, 1)) / 1e3
     alg_bytes = 2.0 * bpv * dofv  # one read + one write of f per update
     achieved = alg_bytes / avg_upd_s / 1e9 if upd_n else None
-    traffic = _traffic_from_profiles(f"cylinder{E}x{V}", args.precision)
+    traffic = _traffic_from_profiles(f"{args.config}:{E}x{V}", args.precision)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -106,7 +135,7 @@ def main():
         tc = time.perf_counter() - tc0
         cpu = {"value": dofv * args.cpu_steps / tc, "unit": "DOF-velocity updates/s",
                "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same cylinder workload "
+               "sample": f"{args.cpu_steps} steps of the same {args.config} workload "
                          f"(E={E}, V={V}), FP64 C oracle, reference order, {tc:.1f} s"}
         o.close()
 
@@ -122,10 +151,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32 (moments f64)",
-        "data": "synthetic (seeded graded Delaunay cylinder mesh, freestream Maxwellian start)",
-        "config": {"workload": "cylinder flow, plain method (BASELINE config 2)",
+        "data": "synthetic (seeded graded Delaunay mesh, freestream Maxwellian start)",
+        "config": {"workload": wl,
                    "nelem": E, "npoin": case.mesh.npoin, "ordinates": V,
-                   "rorder": case.rorder, "torder": case.torder, "mach": 0.5, "kn": 0.05,
+                   "rorder": case.rorder, "torder": case.torder, "mach": mach, "kn": kn,
                    "order": "in-place velocity sweep", "precision": args.precision},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,

@@ -96,15 +96,122 @@ __device__ __forceinline__ void nsum(double& s, double& c, double x) {
 // ---------------------------------------------------------------------------
 // k_update: one block = one tile of <= 256 elements (owned + halo) for a
 // chunk of ordinates.  Per ordinate: phase A computes every slot's predictor
-// (GTEQNF + GETELC's UMEAN, src/getelc.f90:95-138) into LDS; phase B gives
-// each owned element its volume term, its three upwind edge fluxes
-// (element-centric EDGFLXOPT: each element recomputes the flux of its own
-// sides, so there is no scatter and no atomic), GETINC, ADTHEM and INFLOW.
+// (GTEQNF + GETELC's UMEAN, src/getelc.f90:95-138) and stages f and U in
+// LDS; phase B gives each owned element its volume term, its upwind edge
+// fluxes (element-centric EDGFLXOPT: each element recomputes the flux of its
+// own sides, so there is no scatter and no atomic), GETINC, ADTHEM and
+// INFLOW.  Wall sides are left to k_wall: wall elements hand their partial
+// right-hand side to it through wpart.
 // ---------------------------------------------------------------------------
-template <typename T, bool kGS>
+
+// What one slot keeps in registers over all ordinates of its block.
+template <typename T>
+struct SlotState {
+  T gx[3], gy[3], pA[3], pB[3], pU[3], pV[3], pM[3];
+  T area, mmat, nx[3], ny[3], al[3];
+  int info[3], mask, wsl, e;
+  bool owned, active;
+};
+
+template <typename T>
+__device__ __forceinline__ void store4(T* p, T a, T b, T c, T d);
+template <>
+__device__ __forceinline__ void store4<float>(float* p, float a, float b, float c, float d) {
+  *reinterpret_cast<float4*>(p) = make_float4(a, b, c, d);
+}
+template <>
+__device__ __forceinline__ void store4<double>(double* p, double a, double b, double c,
+                                                   double d) {
+  reinterpret_cast<double2*>(p)[0] = make_double2(a, b);
+  reinterpret_cast<double2*>(p)[1] = make_double2(c, d);
+}
+
+// One ordinate v of one slot.  f: the slot's three node values at v; sf:
+// this ordinate's LDS buffer, [slot][4] = f at the three nodes and U.
+// Contains the block's one barrier per ordinate, so every thread of the
+// block calls it for every v.
+template <typename T>
+__device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotState<T>& s,
+                                                int v, T ux, T uy, T fin, const T (&f)[3],
+                                                T* __restrict__ sf, T dte, T& res, T& chg) {
+  const int slot = threadIdx.x;
+  const size_t N3 = 3 * (size_t)a.E;
+  const T c13 = T(1) / T(3);
+  T coll[3] = {}, U = 0;
+  if (s.active) {
+    T dive = 0, cs = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const T dx = s.pU[k] - ux, dy = s.pV[k] - uy;
+      const T feq = s.pA[k] * dexp(-s.pB[k] * (dx * dx + dy * dy));
+      coll[k] = s.pM[k] * (feq - f[k]);
+      dive += s.gx[k] * (ux * f[k]) + s.gy[k] * (uy * f[k]);
+      cs += coll[k];
+    }
+    dive = dive - c13 * cs;
+    U = (dive * dte) * T(0.5);
+    store4(sf + 4 * slot, f[0], f[1], f[2], U);
+  }
+  __syncthreads();
+  if (!s.owned) return;
+  // volume term (src/getelc.f90:142-172)
+  const T fxme = c13 * (ux * (f[0] - U) + ux * (f[1] - U) + ux * (f[2] - U));
+  const T fyme = c13 * (uy * (f[0] - U) + uy * (f[1] - U) + uy * (f[2] - U));
+  T rhs[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) rhs[k] = (s.gx[k] * s.area) * fxme + (s.gy[k] * s.area) * fyme;
+  // edges (src/edgflxopt.f90:72-125, src/getbou.f90:142-187).  Local edge k
+  // runs from node k to node k1 = (k+1)%3 with this element's outward
+  // normal.  The reference orients each side once (left element) and
+  // subtracts the left element's flux from the right element; negating the
+  // normal is exact in floating point, so the right element's outward form
+  // gives the same values.  Upwind: own values when c.n > 0 (on a tie the
+  // right element's values, as in the reference), always on a boundary
+  // side; inflow and wall sides have zero length here.
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int k1 = (k + 1) % 3;
+    const int in = s.info[k];
+    const T cn = ux * s.nx[k] + uy * s.ny[k];
+    const bool own = (in & kUpdOwn) || cn > T(0) || ((in & kUpdRight) && cn == T(0));
+    const int src = own ? slot : (in >> kUpdSlot);
+    const int j1 = own ? k : ((in >> kUpdNbK) & 3);
+    const int j2 = own ? k1 : ((in >> kUpdNbK1) & 3);
+    const T* q = sf + 4 * src;
+    const T us = q[3];
+    const T g1 = q[j1] - us, g2 = q[j2] - us;
+    T r1, r2;
+    edge_rhsi(ux, uy, s.nx[k], s.ny[k], s.al[k], g1, g2, r1, r2);
+    rhs[k] += r1;
+    rhs[k1] += r2;
+  }
+  if (s.wsl >= 0) {
+    T* wp = a.wpart + ((size_t)s.wsl * a.V + v) * 8;
+    store4(wp, rhs[0], rhs[1], rhs[2], coll[0]);
+    store4(wp + 4, coll[1], coll[2], U, T(0));
+    return;
+  }
+  // GETINC + ADTHEM + INFLOW (src/getinc.f90:30-42, src/adthem.f90:109-118,
+  // src/inflow.f90:165-181)
+  T* fo = a.fnxt + (size_t)v * N3 + 3 * (size_t)s.e;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const T d = s.mmat * (dte * rhs[k]) + coll[k] * dte;
+    res = fmax(res, fabs(d));
+    T fn = f[k] + d;
+    fn = fn < T(0) ? T(0) : fn;
+    if ((s.mask >> k) & 1) fn = fin;
+    chg = fmax(chg, fabs(fn - f[k]));
+    fo[k] = fn;
+  }
+}
+
+// kDepth: ordinates whose f is in flight ahead of the one being computed.
+// The queue is indexed by compile-time slots of an unrolled loop, so no
+// register copy ever waits on a load that is still outstanding.
+template <typename T, int kDepth>
 __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
-  __shared__ T s_f[2][kSlots * 3];
-  __shared__ T s_u[2][kSlots];
+  __shared__ __align__(16) T s_f[2][kSlots * 4];
   __shared__ double s_red[8];
   const int logical = xcd_logical(blockIdx.x, gridDim.x);
   const int tile = logical % a.ntiles;
@@ -113,184 +220,112 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
   const int e0 = a.tiles[4 * tile], nown = a.tiles[4 * tile + 1];
   const int hoff = a.tiles[4 * tile + 2], nh = a.tiles[4 * tile + 3];
   const int slot = threadIdx.x;
-  const bool owned = slot < nown;
-  const bool active = slot < nown + nh;
-  const int e = owned ? e0 + slot : (active ? a.halo[hoff + slot - nown] : 0);
   const size_t N3 = 3 * (size_t)a.E;
   const int E = a.E, V = a.V;
 
-  T gx[3] = {}, gy[3] = {}, pA[3] = {}, pB[3] = {}, pU[3] = {}, pV[3] = {}, pM[3] = {};
-  if (active) {
+  SlotState<T> s;
+  s.owned = slot < nown;
+  s.active = slot < nown + nh;
+  s.e = s.owned ? e0 + slot : (s.active ? a.halo[hoff + slot - nown] : 0);
+  const int e = s.e;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    s.gx[k] = s.gy[k] = s.pA[k] = s.pB[k] = s.pU[k] = s.pV[k] = s.pM[k] = T(0);
+    s.nx[k] = s.ny[k] = s.al[k] = T(0);
+    s.info[k] = kUpdOwn;
+  }
+  s.area = s.mmat = T(0);
+  s.mask = 0;
+  s.wsl = -1;
+  if (s.active) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      gx[k] = a.geo[k * E + e];
-      gy[k] = a.geo[(3 + k) * E + e];
+      s.gx[k] = a.geo[k * E + e];
+      s.gy[k] = a.geo[(3 + k) * E + e];
       const size_t nd = 3 * (size_t)e + k;
-      pA[k] = a.np[nd];
-      pB[k] = a.np[N3 + nd];
-      pU[k] = a.np[2 * N3 + nd];
-      pV[k] = a.np[3 * N3 + nd];
-      pM[k] = a.np[4 * N3 + nd];
+      s.pA[k] = a.np[nd];
+      s.pB[k] = a.np[N3 + nd];
+      s.pU[k] = a.np[2 * N3 + nd];
+      s.pV[k] = a.np[3 * N3 + nd];
+      s.pM[k] = a.np[4 * N3 + nd];
     }
   }
-  T area = 0, mmat = 0, enx[3] = {}, eny[3] = {}, eal[3] = {};
-  int info[3] = {1, 1, 1}, nsl[3] = {-1, -1, -1}, mask = 0, wsl = -1;
-  if (owned) {
-    area = a.geo[6 * E + e];
-    mmat = a.geo[7 * E + e];
+  if (s.owned) {
+    s.area = a.geo[6 * E + e];
+    s.mmat = a.geo[7 * E + e];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      info[k] = a.einfo[k * E + e];
-      nsl[k] = a.eslot[k * E + e];
-      enx[k] = a.enx[k * E + e];
-      eny[k] = a.eny[k * E + e];
-      eal[k] = a.ealen[k * E + e];
+      s.info[k] = a.uinfo[k * E + e];
+      s.nx[k] = a.unx[k * E + e];
+      s.ny[k] = a.uny[k * E + e];
+      s.al[k] = a.ual[k * E + e];
     }
-    mask = a.emask[e];
-    if (kGS) wsl = a.ewslot[e];
+    s.mask = a.emask[e];
+    s.wsl = a.ewslot[e];
   }
   const T dte = (T)a.scal[0];
-  const T c13 = T(1) / T(3);
-  const T alpha = a.alpha;
-  T res = 0;
+  T res = 0, chg = 0;
 
-  // software pipeline: the next ordinate's f and velocity are loaded while
-  // the current one is computed, so the loads' latency is hidden behind the
-  // barrier-separated phases
-  T fq[3] = {};
-  T uxq = a.vtab[v0], uyq = a.vtab[V + v0];
-  if (active) {
-    const T* fp = a.fcur + (size_t)v0 * N3 + 3 * (size_t)e;
-    fq[0] = fp[0];
-    fq[1] = fp[1];
-    fq[2] = fp[2];
+  // prime the queue: f and the ordinate's velocity for v0 .. v0+kDepth-1
+  T q[kDepth][3], qx[kDepth], qy[kDepth], qi[kDepth];
+  const T* fe = a.fcur + 3 * (size_t)e;
+#pragma unroll
+  for (int j = 0; j < kDepth; ++j) {
+    q[j][0] = q[j][1] = q[j][2] = T(0);
+    qx[j] = qy[j] = qi[j] = T(0);
+    const int v = v0 + j;
+    if (v < v1) {
+      qx[j] = a.vtab[v];
+      qy[j] = a.vtab[V + v];
+      qi[j] = a.vtab[2 * V + v];
+      if (s.active) {
+        const T* fp = fe + (size_t)v * N3;
+        q[j][0] = fp[0];
+        q[j][1] = fp[1];
+        q[j][2] = fp[2];
+      }
+    }
   }
-  for (int v = v0; v < v1; ++v) {
-    const int buf = (v - v0) & 1;
-    const T ux = uxq, uy = uyq;
-    T f[3] = {fq[0], fq[1], fq[2]}, coll[3] = {}, U = 0;
-    if (v + 1 < v1) {
-      uxq = a.vtab[v + 1];
-      uyq = a.vtab[V + v + 1];
-      if (active) {
-        const T* fp = a.fcur + (size_t)(v + 1) * N3 + 3 * (size_t)e;
-        fq[0] = fp[0];
-        fq[1] = fp[1];
-        fq[2] = fp[2];
-      }
-    }
-    if (active) {
-      T dive = 0, cs = 0;
+  for (int vb = v0; vb < v1; vb += kDepth) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const T dx = pU[k] - ux, dy = pV[k] - uy;
-        const T feq = pA[k] * dexp(-pB[k] * (dx * dx + dy * dy));
-        coll[k] = pM[k] * (feq - f[k]);
-        dive += gx[k] * (ux * f[k]) + gy[k] * (uy * f[k]);
-        cs += coll[k];
-      }
-      dive = dive - c13 * cs;
-      U = (dive * dte) * T(0.5);
-      s_f[buf][3 * slot + 0] = f[0];
-      s_f[buf][3 * slot + 1] = f[1];
-      s_f[buf][3 * slot + 2] = f[2];
-      s_u[buf][slot] = U;
-    }
-    __syncthreads();
-    if (!owned) continue;
-    // volume term (src/getelc.f90:142-172)
-    const T fxme = c13 * (ux * (f[0] - U) + ux * (f[1] - U) + ux * (f[2] - U));
-    const T fyme = c13 * (uy * (f[0] - U) + uy * (f[1] - U) + uy * (f[2] - U));
-    T rhs[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) rhs[k] = (gx[k] * area) * fxme + (gy[k] * area) * fyme;
-    // edges (src/edgflxopt.f90:72-125, src/getbou.f90:142-281).  Local edge k
-    // runs from node k to node k1 = (k+1)%3; the side's points p1, p2 are
-    // (k, k1) when this element is the side's left element and (k1, k)
-    // when it is the right one, so every register index is compile-time.
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int k1 = (k + 1) % 3;
-      const int in = info[k];
-      const int kind = in & 3;
-      if (kind == 1) continue;  // inflow side: no flux
-      T g1, g2, r1, r2;
-      if (kind == 0) {
-        const bool left = (in >> 10) & 1;
-        const T scpr = ux * enx[k] + uy * eny[k];
-        if ((scpr > T(0)) == left) {
-          g1 = (left ? f[k] : f[k1]) - U;
-          g2 = (left ? f[k1] : f[k]) - U;
-        } else {
-          const int ns = nsl[k], b1 = (in >> 6) & 3, b2 = (in >> 8) & 3;
-          const T un = s_u[buf][ns];
-          g1 = s_f[buf][3 * ns + b1] - un;
-          g2 = s_f[buf][3 * ns + b2] - un;
+    for (int j = 0; j < kDepth; ++j) {
+      const int v = vb + j;
+      if (v < v1) {  // uniform over the block
+        const T f[3] = {q[j][0], q[j][1], q[j][2]};
+        const T ux = qx[j], uy = qy[j], fin = qi[j];
+        const int vn = v + kDepth;
+        if (vn < v1) {
+          qx[j] = a.vtab[vn];
+          qy[j] = a.vtab[V + vn];
+          qi[j] = a.vtab[2 * V + vn];
+          if (s.active) {
+            const T* fp = fe + (size_t)vn * N3;
+            q[j][0] = fp[0];
+            q[j][1] = fp[1];
+            q[j][2] = fp[2];
+          }
         }
-        edge_rhsi(ux, uy, enx[k], eny[k], eal[k], g1, g2, r1, r2);
-        if (left) { rhs[k] += r1; rhs[k1] += r2; }
-        else { rhs[k1] -= r1; rhs[k] -= r2; }
-        continue;
+        update_ordinate<T>(a, s, v, ux, uy, fin, f, s_f[(v - v0) & 1], dte, res, chg);
       }
-      // boundary side: this element is its left element (p1, p2) = (k, k1)
-      int i1 = -1, ws = -1;
-      if (kind == 3) {
-        ws = nsl[k];
-        i1 = a.refl_i[2 * ((size_t)ws * V + v)];
-      }
-      if (i1 < 0) {  // outflow, or molecules moving into the wall
-        g1 = f[k] - U;
-        g2 = f[k1] - U;
-      } else if (!kGS) {  // reflected distribution, previous-step values
-        const int i2 = a.refl_i[2 * ((size_t)ws * V + v) + 1];
-        const T* rt = a.refl_t + 3 * ((size_t)ws * V + v);
-        const T zd1 = rt[0], zd2 = rt[1], mw = rt[2];
-        const T* fb = a.fcur + 3 * (size_t)e;
-        const T fa1 = fb[(size_t)i1 * N3 + k], fb1 = fb[(size_t)i2 * N3 + k];
-        const T fa2 = fb[(size_t)i1 * N3 + k1], fb2 = fb[(size_t)i2 * N3 + k1];
-        const T rf1 = (zd2 * fa1 + zd1 * fb1) / (zd1 + zd2);
-        const T rf2 = (zd2 * fa2 + zd1 * fb2) / (zd1 + zd2);
-        const T mf = mw * (T)a.eta[a.wall_bside[ws]];
-        g1 = (T(1) - alpha) * rf1 + alpha * mf;
-        g2 = (T(1) - alpha) * rf2 + alpha * mf;
-      } else {
-        continue;  // the ordered wall pass adds it
-      }
-      edge_rhsi(ux, uy, enx[k], eny[k], eal[k], g1, g2, r1, r2);
-      rhs[k] += r1;
-      rhs[k1] += r2;
-    }
-    if (kGS && wsl >= 0) {
-      T* wp = a.wpart + ((size_t)wsl * V + v) * 6;
-      wp[0] = rhs[0]; wp[1] = rhs[1]; wp[2] = rhs[2];
-      wp[3] = coll[0]; wp[4] = coll[1]; wp[5] = coll[2];
-      continue;
-    }
-    // GETINC + ADTHEM + INFLOW (src/getinc.f90:30-42, src/adthem.f90:109-118,
-    // src/inflow.f90:165-181)
-    T* fo = a.fnxt + (size_t)v * N3 + 3 * (size_t)e;
-    const T fin = a.vtab[2 * V + v];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const T d = mmat * (dte * rhs[k]) + coll[k] * dte;
-      res = fmax(res, fabs(d));
-      T fn = f[k] + d;
-      fn = fn < T(0) ? T(0) : fn;
-      if ((mask >> k) & 1) fn = fin;
-      fo[k] = fn;
     }
   }
   const double r = block_max((double)res, s_red);
   if (threadIdx.x == 0) a.res_part[logical] = r;
+  const double rc = block_max((double)chg, s_red);
+  if (threadIdx.x == 0) a.res_part[a.res_stride + logical] = rc;
 }
 
 // ---------------------------------------------------------------------------
-// k_wall: the reference's in-place velocity sweep at wall elements.  The
-// reflected flux of ordinate v reads f at ordinates I1, I2 of the same
-// element; those already advanced this step (index < v) are read from the
-// new buffer (src/getbou.f90:245-250 inside src/advnce.f90:273-383).  One
-// thread per (wall element, speed ring) when every reflection stays in its
-// ring, else per wall element over all ordinates.
+// k_wall: the wall sides of the wall elements, and their update.  Per
+// ordinate, a wall side's flux is the interior form on the element's own
+// values when c.n >= 0 (src/getbou.f90:174-187), else the reflected
+// distribution (src/getbou.f90:190-255).  In the reference's in-place
+// velocity sweep the reflection reads f at ordinates I1, I2 of the same
+// element, and those already advanced this step (index < v) are read from
+// the new buffer (src/getbou.f90:245-250 inside src/advnce.f90:273-383);
+// with jacobi set every read is from the previous step.  One thread per
+// (wall element, speed ring) when every reflection stays in its ring, else
+// per wall element over all ordinates.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
@@ -299,9 +334,9 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
   // ring): the in-place reads of ordinates < v come from LDS
   extern __shared__ unsigned char s_dyn[];
   T* s_ring = reinterpret_cast<T*>(s_dyn) + (size_t)threadIdx.x * 3 * a.T_;
-  const bool use_lds = a.ngroups > 1;
+  const bool use_lds = a.ngroups > 1 && !a.jacobi;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  double res = 0.0;
+  double res = 0.0, chg = 0.0;
   if (gid < a.nwe * a.ngroups) {
     const int we = gid / a.ngroups, grp = gid % a.ngroups;
     const int e = a.wall_elems[we];
@@ -324,49 +359,61 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
     const T dte = (T)a.scal[0];
     const T alpha = a.alpha;
     for (int v = vb; v < ve; ++v) {
-      const T* wp = a.wpart + ((size_t)we * V + v) * 6;
+      const T* wp = a.wpart + ((size_t)we * V + v) * 8;
       T rhs[3] = {wp[0], wp[1], wp[2]};
       const T coll[3] = {wp[3], wp[4], wp[5]};
+      const T U = wp[6];
       const T ux = a.vtab[v], uy = a.vtab[V + v];
+      const T* fc = a.fcur + (size_t)v * N3 + 3 * (size_t)e;
+      const T f[3] = {fc[0], fc[1], fc[2]};
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         if ((info[k] & 3) != 3) continue;
+        const int a1 = k, a2 = (k + 1) % 3;  // boundary side: (p1, p2) = (k, k+1)
         const int ws = nsl[k];
         const size_t rk = (size_t)ws * V + v;
         const int i1 = a.refl_i[2 * rk];
-        if (i1 < 0) continue;
-        const int i2 = a.refl_i[2 * rk + 1];
-        const int a1 = k, a2 = (k + 1) % 3;  // boundary side: (p1, p2) = (k, k+1)
-        const T zd1 = a.refl_t[3 * rk], zd2 = a.refl_t[3 * rk + 1], mw = a.refl_t[3 * rk + 2];
-        const T* s1;
-        const T* s2;
-        if (use_lds) {
-          s1 = i1 < v ? s_ring + 3 * (i1 - vb) : a.fcur + (size_t)i1 * N3 + 3 * (size_t)e;
-          s2 = i2 < v ? s_ring + 3 * (i2 - vb) : a.fcur + (size_t)i2 * N3 + 3 * (size_t)e;
+        T g1, g2;
+        if (i1 < 0) {  // molecules leaving through the wall: own values
+          g1 = f[a1] - U;
+          g2 = f[a2] - U;
         } else {
-          s1 = (i1 < v ? a.fnxt : a.fcur) + (size_t)i1 * N3 + 3 * (size_t)e;
-          s2 = (i2 < v ? a.fnxt : a.fcur) + (size_t)i2 * N3 + 3 * (size_t)e;
+          const int i2 = a.refl_i[2 * rk + 1];
+          const T zd1 = a.refl_t[3 * rk], zd2 = a.refl_t[3 * rk + 1];
+          const T mw = a.refl_t[3 * rk + 2];
+          const T* s1;
+          const T* s2;
+          if (use_lds) {
+            s1 = i1 < v ? s_ring + 3 * (i1 - vb) : a.fcur + (size_t)i1 * N3 + 3 * (size_t)e;
+            s2 = i2 < v ? s_ring + 3 * (i2 - vb) : a.fcur + (size_t)i2 * N3 + 3 * (size_t)e;
+          } else if (a.jacobi) {
+            s1 = a.fcur + (size_t)i1 * N3 + 3 * (size_t)e;
+            s2 = a.fcur + (size_t)i2 * N3 + 3 * (size_t)e;
+          } else {
+            s1 = (i1 < v ? a.fnxt : a.fcur) + (size_t)i1 * N3 + 3 * (size_t)e;
+            s2 = (i2 < v ? a.fnxt : a.fcur) + (size_t)i2 * N3 + 3 * (size_t)e;
+          }
+          const T rf1 = (zd2 * s1[a1] + zd1 * s2[a1]) / (zd1 + zd2);
+          const T rf2 = (zd2 * s1[a2] + zd1 * s2[a2]) / (zd1 + zd2);
+          const T mf = mw * (T)a.eta[a.wall_bside[ws]];
+          g1 = (T(1) - alpha) * rf1 + alpha * mf;
+          g2 = (T(1) - alpha) * rf2 + alpha * mf;
         }
-        const T rf1 = (zd2 * s1[a1] + zd1 * s2[a1]) / (zd1 + zd2);
-        const T rf2 = (zd2 * s1[a2] + zd1 * s2[a2]) / (zd1 + zd2);
-        const T mf = mw * (T)a.eta[a.wall_bside[ws]];
-        const T g1 = (T(1) - alpha) * rf1 + alpha * mf;
-        const T g2 = (T(1) - alpha) * rf2 + alpha * mf;
         T r1, r2;
         edge_rhsi(ux, uy, enx[k], eny[k], eal[k], g1, g2, r1, r2);
         rhs[a1] += r1;
         rhs[a2] += r2;
       }
-      const T* fc = a.fcur + (size_t)v * N3 + 3 * (size_t)e;
       T* fo = a.fnxt + (size_t)v * N3 + 3 * (size_t)e;
       const T fin = a.vtab[2 * V + v];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const T d = mmat * (dte * rhs[k]) + coll[k] * dte;
         res = fmax(res, (double)fabs(d));
-        T fn = fc[k] + d;
+        T fn = f[k] + d;
         fn = fn < T(0) ? T(0) : fn;
         if ((mask >> k) & 1) fn = fin;
+        chg = fmax(chg, (double)fabs(fn - f[k]));
         fo[k] = fn;
         if (use_lds) s_ring[3 * (v - vb) + k] = fn;
       }
@@ -374,6 +421,8 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
   }
   const double r = block_max(res, s_red);
   if (threadIdx.x == 0) a.res_part[blockIdx.x] = r;
+  const double rc = block_max(chg, s_red);
+  if (threadIdx.x == 0) a.res_part[a.res_stride + blockIdx.x] = rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -562,8 +611,11 @@ __global__ void __launch_bounds__(256) k_final(const double* __restrict__ res_pa
                                                StepRec* __restrict__ rec, int rec_cap,
                                                int record) {
   __shared__ double s_red[8];
-  double r = 0.0, m = 0.0, ms = 0.0, lift = 0.0, drag = 0.0;
-  for (int i = threadIdx.x; i < nres; i += blockDim.x) r = fmax(r, res_part[i]);
+  double r = 0.0, rc = 0.0, m = 0.0, ms = 0.0, lift = 0.0, drag = 0.0;
+  for (int i = threadIdx.x; i < nres; i += blockDim.x) {
+    r = fmax(r, res_part[i]);
+    rc = fmax(rc, res_part[nres + i]);
+  }
   for (int i = threadIdx.x; i < nmom; i += blockDim.x) {
     m = fmax(m, mu_part[i]);
     ms += mass_part[i];
@@ -580,6 +632,7 @@ __global__ void __launch_bounds__(256) k_final(const double* __restrict__ res_pa
     drag -= psav * len * (px / mag);
   }
   r = block_max(r, s_red);
+  rc = block_max(rc, s_red);
   m = block_max(m, s_red);
   ms = block_sum(ms, s_red);
   lift = block_sum(lift, s_red);
@@ -597,6 +650,7 @@ __global__ void __launch_bounds__(256) k_final(const double* __restrict__ res_pa
       rec[slot].mass = ms;
       rec[slot].lift = lift;
       rec[slot].drag = drag;
+      rec[slot].change = rc;
     }
     scal[2] = lift;
     scal[3] = drag;
@@ -695,8 +749,9 @@ struct Ctx {
   void* f[2] = {nullptr, nullptr};
   int cur = 0;
   void *geo = nullptr, *enx = nullptr, *eny = nullptr, *ealen = nullptr, *np = nullptr,
+       *unx = nullptr, *uny = nullptr, *ual = nullptr,
        *vtab = nullptr, *refl_t = nullptr, *wpart = nullptr;
-  int *einfo = nullptr, *eslot = nullptr, *emask = nullptr, *ewslot = nullptr,
+  int *einfo = nullptr, *eslot = nullptr, *emask = nullptr, *ewslot = nullptr, *uinfo = nullptr,
       *refl_i = nullptr, *wall_bside = nullptr, *tiles = nullptr, *halo = nullptr,
       *wall_elems = nullptr, *eta_side = nullptr, *fside = nullptr, *pt_ptr = nullptr,
       *pt_node = nullptr, *intma = nullptr;
@@ -706,6 +761,7 @@ struct Ctx {
   StepRec* rec = nullptr;
   int rec_cap = 0;
   int wall_tpb = 256;
+  int depth = 1;  // update kernel: ordinates of f in flight (1..4)
   int ntiles = 0, vchunk = 1, nvchunks = 1, nblk_upd = 0, nblk_wall = 0, ngroups = 1,
       nblk_mom = 0, neta = 0, nforce = 0;
   int itime = 0;
@@ -794,6 +850,17 @@ void tend(Ctx& c, int k) {
 }
 
 template <typename T>
+void launch_update(Ctx& c, const UpdArgs<T>& u) {
+  const dim3 g(c.nblk_upd), b(kSlots);
+  switch (c.depth) {
+    case 1: hipLaunchKernelGGL((k_update<T, 1>), g, b, 0, c.stream, u); break;
+    case 2: hipLaunchKernelGGL((k_update<T, 2>), g, b, 0, c.stream, u); break;
+    case 3: hipLaunchKernelGGL((k_update<T, 3>), g, b, 0, c.stream, u); break;
+    default: hipLaunchKernelGGL((k_update<T, 4>), g, b, 0, c.stream, u); break;
+  }
+}
+
+template <typename T>
 int step_once(Ctx& c) {
   const Prep& p = c.p;
   const T* fc = (const T*)c.f[c.cur];
@@ -806,18 +873,14 @@ int step_once(Ctx& c) {
     tend(c, 3);
   }
   UpdArgs<T> u;
-  u.fcur = fc; u.fnxt = fn; u.geo = (const T*)c.geo; u.einfo = c.einfo; u.eslot = c.eslot;
-  u.enx = (const T*)c.enx; u.eny = (const T*)c.eny; u.ealen = (const T*)c.ealen;
+  u.fcur = fc; u.fnxt = fn; u.geo = (const T*)c.geo; u.uinfo = c.uinfo;
+  u.unx = (const T*)c.unx; u.uny = (const T*)c.uny; u.ual = (const T*)c.ual;
   u.emask = c.emask; u.ewslot = c.ewslot; u.np = (const T*)c.np; u.vtab = (const T*)c.vtab;
-  u.refl_i = c.refl_i; u.refl_t = (const T*)c.refl_t; u.wall_bside = c.wall_bside;
-  u.eta = c.eta; u.tiles = c.tiles; u.halo = c.halo; u.wpart = (T*)c.wpart;
-  u.scal = c.scal; u.res_part = c.res_part; u.alpha = (T)p.prm.alpha;
+  u.tiles = c.tiles; u.halo = c.halo; u.wpart = (T*)c.wpart;
+  u.scal = c.scal; u.res_part = c.res_part; u.res_stride = c.nblk_upd + c.nblk_wall;
   u.E = p.E; u.V = p.V; u.ntiles = c.ntiles; u.vchunk = c.vchunk;
   tbegin(c, 0);
-  if (c.gs && !p.wall_elems.empty())
-    hipLaunchKernelGGL((k_update<T, true>), dim3(c.nblk_upd), dim3(kSlots), 0, c.stream, u);
-  else
-    hipLaunchKernelGGL((k_update<T, false>), dim3(c.nblk_upd), dim3(kSlots), 0, c.stream, u);
+  launch_update<T>(c, u);
   HIPCK(hipGetLastError());
   tend(c, 0);
   if (c.nblk_wall) {
@@ -827,8 +890,10 @@ int step_once(Ctx& c) {
     w.emask = c.emask; w.vtab = (const T*)c.vtab; w.refl_i = c.refl_i;
     w.refl_t = (const T*)c.refl_t; w.wall_bside = c.wall_bside; w.eta = c.eta;
     w.wall_elems = c.wall_elems; w.wpart = (const T*)c.wpart; w.scal = c.scal;
-    w.res_part = c.res_part + c.nblk_upd; w.alpha = (T)p.prm.alpha; w.E = p.E; w.V = p.V;
+    w.res_part = c.res_part + c.nblk_upd; w.res_stride = c.nblk_upd + c.nblk_wall;
+    w.alpha = (T)p.prm.alpha; w.E = p.E; w.V = p.V;
     w.T_ = p.T; w.ngroups = c.ngroups; w.nwe = (int)p.wall_elems.size();
+    w.jacobi = c.gs ? 0 : 1;
     tbegin(c, 1);
     const size_t lds = c.ngroups > 1 ? (size_t)c.wall_tpb * 3 * p.T * sizeof(T) : 0;
     hipLaunchKernelGGL(k_wall<T>, dim3(c.nblk_wall), dim3(c.wall_tpb), lds, c.stream, w);
@@ -902,6 +967,8 @@ int upload_tables(Ctx& c) {
   const size_t N3 = 3 * (size_t)E;
   std::vector<T> geo(8 * (size_t)E), enx(3 * (size_t)E), eny(3 * (size_t)E), eal(3 * (size_t)E);
   std::vector<int> einfo(3 * (size_t)E), eslot(3 * (size_t)E), emask(E), ewslot(E);
+  std::vector<T> unx(3 * (size_t)E), uny(3 * (size_t)E), ual(3 * (size_t)E);
+  std::vector<int> uinfo(3 * (size_t)E);
   for (int e = 0; e < E; ++e) {
     const double* g = &p.geome[7 * (size_t)e];
     for (int k = 0; k < 6; ++k) geo[(size_t)k * E + e] = (T)g[k];
@@ -917,6 +984,21 @@ int upload_tables(Ctx& c) {
       enx[i] = (T)ed.nx;
       eny[i] = (T)ed.ny;
       eal[i] = (T)ed.alen;
+      // update kernel: outward normal (the right element's is the side's
+      // normal negated), the neighbour's nodes at this edge's nodes k and
+      // k+1, and no flux through inflow and wall sides
+      const T sg = ed.sign > 0 ? T(1) : T(-1);
+      unx[i] = sg * (T)ed.nx;
+      uny[i] = sg * (T)ed.ny;
+      if (ed.kind == kInterior) {
+        const int nbk = ed.sign > 0 ? ed.b1 : ed.b2, nbk1 = ed.sign > 0 ? ed.b2 : ed.b1;
+        uinfo[i] = (nbk << kUpdNbK) | (nbk1 << kUpdNbK1) | (ed.sign > 0 ? 0 : kUpdRight) |
+                   (p.nbr_slot[3 * (size_t)e + k] << kUpdSlot);
+        ual[i] = (T)ed.alen;
+      } else {
+        uinfo[i] = kUpdOwn;
+        ual[i] = ed.kind == kOutflow ? (T)ed.alen : T(0);
+      }
     }
     emask[e] = p.inflow_mask[e];
     ewslot[e] = p.wall_elem_slot[e];
@@ -930,6 +1012,10 @@ int upload_tables(Ctx& c) {
   if ((st = upload(&c.eslot, eslot, c.owned))) return st;
   if ((st = upload(&c.emask, emask, c.owned))) return st;
   if ((st = upload(&c.ewslot, ewslot, c.owned))) return st;
+  if ((st = upload((T**)&c.unx, unx, c.owned))) return st;
+  if ((st = upload((T**)&c.uny, uny, c.owned))) return st;
+  if ((st = upload((T**)&c.ual, ual, c.owned))) return st;
+  if ((st = upload(&c.uinfo, uinfo, c.owned))) return st;
   std::vector<T> vtab(3 * (size_t)V);
   std::vector<double> vtd(5 * (size_t)V);
   for (int v = 0; v < V; ++v) {
@@ -1000,7 +1086,7 @@ int upload_tables(Ctx& c) {
   if ((st = alloc(&c.pts, 6 * (size_t)p.P, c.owned))) return st;
   if ((st = alloc(&c.ps_glob, (size_t)p.P, c.owned))) return st;
   if ((st = alloc(&c.ps4, 4 * (size_t)p.P, c.owned))) return st;
-  if ((st = alloc((T**)&c.wpart, c.gs ? p.wall_elems.size() * (size_t)V * 6 : 1, c.owned)))
+  if ((st = alloc((T**)&c.wpart, std::max<size_t>(1, p.wall_elems.size() * (size_t)V * 8), c.owned)))
     return st;
   for (int b = 0; b < 2; ++b)
     if ((st = alloc((T**)&c.f[b], (size_t)V * N3, c.owned))) return st;
@@ -1025,7 +1111,7 @@ int refresh(Ctx& c) {
   if (st) return st;
   if ((st = launch_points(c, 1))) return st;
   // DTE for the next step, no record
-  HIPCK(hipMemsetAsync(c.res_part, 0, sizeof(double) * (c.nblk_upd + c.nblk_wall), c.stream));
+  HIPCK(hipMemsetAsync(c.res_part, 0, 2 * sizeof(double) * (c.nblk_upd + c.nblk_wall), c.stream));
   return launch_final(c, -1);
 }
 
@@ -1131,6 +1217,14 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   c.gs = prm->order == DGBGK_ORDER_REFERENCE ? 1 : 0;
   if (const char* ng = std::getenv("DGBGK_NO_GRAPH")) c.use_graph = ng[0] == '0';
   c.tsz = c.prec ? 4 : 8;
+  // ordinates of f in flight per update thread beyond the one computed:
+  // one is fastest on gfx950 (deeper queues cost more in occupancy than
+  // they gain in latency hiding)
+  c.depth = 1;
+  if (const char* pd = std::getenv("DGBGK_PREFETCH")) {
+    const int d = std::atoi(pd);
+    if (d >= 1 && d <= 4) c.depth = d;
+  }
   const Prep& p = c.p;
   auto cleanup = [&](int code) {
     for (void* q : c.owned) hipFree(q);
@@ -1163,13 +1257,14 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
     while (c.wall_tpb > 64 && (size_t)c.wall_tpb * per > 65536) c.wall_tpb >>= 1;
     if ((size_t)c.wall_tpb * per > 65536) c.ngroups = 1;  // ring too long: no LDS staging
   }
-  c.nblk_wall = (c.gs && nwe) ? (nwe * c.ngroups + c.wall_tpb - 1) / c.wall_tpb : 0;
+  c.nblk_wall = nwe ? (nwe * c.ngroups + c.wall_tpb - 1) / c.wall_tpb : 0;
   c.nblk_mom = (int)((3 * (size_t)p.E + 255) / 256);
   c.neta = (int)p.eta_side.size();
   c.nforce = (int)p.force_p1.size();
   st = c.prec ? upload_tables<float>(c) : upload_tables<double>(c);
   if (st) return cleanup(st);
-  if ((st = alloc(&c.res_part, (size_t)c.nblk_upd + c.nblk_wall, c.owned))) return cleanup(st);
+  if ((st = alloc(&c.res_part, 2 * ((size_t)c.nblk_upd + c.nblk_wall), c.owned)))
+    return cleanup(st);
   if ((st = alloc(&c.mu_part, (size_t)c.nblk_mom, c.owned))) return cleanup(st);
   if ((st = alloc(&c.mass_part, (size_t)c.nblk_mom, c.owned))) return cleanup(st);
   std::vector<double> scal(8, 0.0);

@@ -5,7 +5,9 @@
 //   geo      T[8][E]    dN/dx (3), dN/dy (3), area, 1/lumped mass  (SoA)
 //   einfo    int[3][E]  per element edge: kind | a1 | a2 | b1 | b2 | left
 //   eslot    int[3][E]  interior: neighbour's LDS slot; wall: wall-side slot
-//   enx/eny/ealen T[3][E] flux normal and length/6 of each element edge
+//   enx/eny/ealen T[3][E] side normal and length/6 of each element edge
+//   uinfo/unx/uny/ual    the same edges for the update kernel: outward
+//                        normal, neighbour slot and nodes (see kUpd*)
 //   np       T[5][3E]   per node: n*beta2/pi, beta2, u, v, mu (GTEQNF/COMPMU)
 //   vtab     T[3][V]    ordinate velocity (ADVNCE form) and inflow value
 //   mom      double[5][3E] DISND, DISUX, DISUY, DISPS, MU
@@ -23,34 +25,38 @@ __host__ __device__ constexpr int einfo_pack(int kind, int a1, int a2, int b1, i
   return kind | (a1 << 2) | (a2 << 4) | (b1 << 6) | (b2 << 8) | (left << 10);
 }
 
-struct StepRec {
-  double sumres, logres, dte, mass, lift, drag;
+struct StepRec {  // same layout as dgbgk_step_info
+  double sumres, logres, dte, mass, lift, drag, change;
 };
+
+// Update-kernel edge word (uinfo), one per element edge k (local nodes k,
+// k1 = (k+1)%3): the flux source when the neighbour is upwind and the
+// tie rule of the side's orientation.
+constexpr int kUpdNbK = 0;      // bits 0-1: neighbour's local node at node k
+constexpr int kUpdNbK1 = 2;     // bits 2-3: neighbour's local node at node k1
+constexpr int kUpdRight = 1 << 4;  // this element is the side's right element
+constexpr int kUpdOwn = 1 << 5;    // boundary side: own values are always upwind
+constexpr int kUpdSlot = 8;     // bits 8+: neighbour's LDS slot
 
 template <typename T>
 struct UpdArgs {
   const T* __restrict__ fcur;
   T* __restrict__ fnxt;
   const T* __restrict__ geo;
-  const int* __restrict__ einfo;
-  const int* __restrict__ eslot;
-  const T* __restrict__ enx;
-  const T* __restrict__ eny;
-  const T* __restrict__ ealen;
+  const int* __restrict__ uinfo;   // [3][E] edge words
+  const T* __restrict__ unx;       // [3][E] outward unit normal of the edge
+  const T* __restrict__ uny;
+  const T* __restrict__ ual;       // [3][E] length/6; 0 on inflow and wall sides
   const int* __restrict__ emask;   // inflow bits 0..2
   const int* __restrict__ ewslot;  // wall element slot or -1
   const T* __restrict__ np;
   const T* __restrict__ vtab;
-  const int* __restrict__ refl_i;  // [NW][V][2]
-  const T* __restrict__ refl_t;    // [NW][V][3] zd1, zd2, mw
-  const int* __restrict__ wall_bside;
-  const double* __restrict__ eta;  // per boundary side
   const int* __restrict__ tiles;   // [ntiles][4] e0, nown, hoff, nhalo
   const int* __restrict__ halo;
-  T* __restrict__ wpart;           // [NWE][V][6] partial RHS and COLL
+  T* __restrict__ wpart;           // [NWE][V][8] partial RHS (3), COLL (3), U
   const double* __restrict__ scal; // [0] = DTE
-  double* __restrict__ res_part;
-  T alpha;
+  double* __restrict__ res_part;   // per block: max|increment|; +res_stride: max|change|
+  int res_stride;
   int E, V, ntiles, vchunk;
 };
 
@@ -74,8 +80,10 @@ struct WallArgs {
   const T* __restrict__ wpart;
   const double* __restrict__ scal;
   double* __restrict__ res_part;
+  int res_stride;
   T alpha;
   int E, V, T_, ngroups, nwe;
+  int jacobi;  // reflections read f of the previous step only
 };
 
 template <typename T>

@@ -54,7 +54,8 @@ class Params(C.Structure):
 
 class StepInfo(C.Structure):
     _fields_ = [("sumres", C.c_double), ("logres", C.c_double), ("dte", C.c_double),
-                ("mass", C.c_double), ("lift", C.c_double), ("drag", C.c_double)]
+                ("mass", C.c_double), ("lift", C.c_double), ("drag", C.c_double),
+                ("change", C.c_double)]
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

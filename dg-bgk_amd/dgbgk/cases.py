@@ -88,27 +88,29 @@ def rv_for(u0: float) -> float:
     return math.ceil(2.0 * (5.0 + abs(u0))) / 2.0
 
 
-def config1(n: int = 32, rorder: int = 8, torder: int = 16) -> Case:
+def unit_square_channel(n: int = 32, rorder: int = 8, torder: int = 16) -> Case:
     """Unit square, 2n^2 P1 elements; 16 angles x 8 speeds on (0,4];
     rho=1, T=1, u=(0.1,0); inflow left, outflow right, specular walls top
     and bottom (type 3, alpha=0); Kn=0.1; 200 steps at CFL 0.5."""
     eta, w = midpoint_speeds(rorder)
     m = meshgen.unit_square(n, left=1, right=2, bottom=3, top=3, tw=1, alpha100=0)
     prm = base_params(u0=0.1, kn=0.1, length=1.0, rv=4.0, alpha=0.0)
-    return Case("c1_unit_square", m, eta, w, torder, prm, nsteps=200)
+    return Case("unit_square_channel", m, eta, w, torder, prm, nsteps=200)
 
 
-def cylinder_mesh(target_elems: int = 100_000, seed: int = 1) -> meshgen.Mesh:
-    """20D x 10D box, cylinder of diameter D=1 at (5,5), graded Delaunay."""
+def cylinder_mesh(target_elems: int = 100_000, seed: int = 1,
+                  far_types=(1, 2, 2, 2)) -> meshgen.Mesh:
+    """20D x 10D box, cylinder of diameter D=1 at (5,5), graded Delaunay.
+    far_types: side types of the (left, right, bottom, top) box sides."""
     hmax = 0.068 * math.sqrt(100_000 / target_elems)
     h0 = min(0.02 * math.sqrt(100_000 / target_elems), 0.5 * hmax)
     return meshgen.body_in_box(20.0, 10.0, 5.0, 5.0, 1.0, h0=h0, growth=0.12,
                                hmax=hmax, seed=seed, wall_type=4, tw=1,
-                               alpha100=100, far_types=(1, 2, 2, 2),
+                               alpha100=100, far_types=tuple(far_types),
                                name=f"cylinder_{target_elems}")
 
 
-def config2(target_elems: int = 100_000, rorder: int = 16, torder: int = 32,
+def cylinder_flow(target_elems: int = 100_000, rorder: int = 16, torder: int = 32,
             mach: float = 0.5, kn: float = 0.05) -> Case:
     """Cylinder flow: Mach 0.5, Kn 0.05 (on D), diffuse isothermal wall T=1,
     512 ordinates, 2000 steps."""
@@ -116,10 +118,10 @@ def config2(target_elems: int = 100_000, rorder: int = 16, torder: int = 32,
     u0 = mach_to_u(mach)
     eta, w = gauss_legendre_speeds(rorder)
     prm = base_params(u0=u0, kn=kn, length=1.0, rv=rv_for(u0), alpha=1.0)
-    return Case("c2_cylinder", m, eta, w, torder, prm, nsteps=2000)
+    return Case("cylinder_flow", m, eta, w, torder, prm, nsteps=2000)
 
 
-def config5(target_elems: int = 1_000_000, rorder: int = 32, torder: int = 64,
+def particle_drag(target_elems: int = 1_000_000, rorder: int = 32, torder: int = 64,
             mach: float = 0.3, kn: float = 1.0) -> Case:
     """Particle of diameter D in a 200D x 100D box, ~1M graded triangles,
     2048 ordinates, Mach 0.3, Kn 1."""
@@ -131,7 +133,7 @@ def config5(target_elems: int = 1_000_000, rorder: int = 32, torder: int = 64,
     u0 = mach_to_u(mach)
     eta, w = gauss_legendre_speeds(rorder)
     prm = base_params(u0=u0, kn=kn, length=1.0, rv=rv_for(u0), alpha=1.0)
-    return Case("c5_particle", m, eta, w, torder, prm, nsteps=100)
+    return Case("particle_drag", m, eta, w, torder, prm, nsteps=100)
 
 
 def lobatto_case(mesh: meshgen.Mesh, eta, w, torder: int, alpha: float = 0.9,

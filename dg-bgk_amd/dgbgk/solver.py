@@ -130,7 +130,7 @@ class Solver:
         info = (_lib.StepInfo * n)()
         check(lib().dgbgk_step(self.h, n, C.cast(info, C.c_void_p)))
         return [dict(sumres=i.sumres, logres=i.logres, dte=i.dte, mass=i.mass, lift=i.lift,
-                     drag=i.drag) for i in info]
+                     drag=i.drag, change=i.change) for i in info]
 
     def get_f(self) -> np.ndarray:
         """f in the reference layout DISNF(3,V,E) as an (E,V,3) array."""

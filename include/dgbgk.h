@@ -68,6 +68,11 @@ typedef struct dgbgk_step_info {
   double mass;   /* sum over DG nodes of DISND*RJ/6 after the step */
   double lift;   /* GETFOR on the last gathered point pressure, */
   double drag;   /* src/advnce.f90:431 (refreshed every 1000 steps) */
+  double change; /* max |f^{k+1} - f^k| over all nodes and ordinates, after
+                    the clamp and the inflow reset (no reference counterpart:
+                    SUMRES counts the increment before the reset, so at
+                    nodes held at the inflow state it never vanishes; this
+                    is the steady-state measure) */
 } dgbgk_step_info;
 
 typedef struct dgbgk_ctx dgbgk_ctx;

@@ -10,12 +10,12 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 def c1(n=32):
     """Config 1 (unit square, specular walls, 8 x 16 ordinates)."""
-    return cases.config1(n=n)
+    return cases.unit_square_channel(n=n)
 
 
 def cylinder_small(elems=2000, rorder=8, torder=16):
     """Cylinder flow with diffuse type-4 walls (FLUCON2 and reflections)."""
-    return cases.config2(target_elems=elems, rorder=rorder, torder=torder)
+    return cases.cylinder_flow(target_elems=elems, rorder=rorder, torder=torder)
 
 
 def lobatto10_walls(n=12, alpha=0.9):
@@ -30,7 +30,7 @@ def lobatto10_walls(n=12, alpha=0.9):
 
 def cylinder_lobatto10(elems=1500, alpha=0.9):
     eta, w = io.read_lobatto(os.path.join(GOLDEN, "Lobatto10.txt"))
-    c = cases.config2(target_elems=elems, rorder=10, torder=10)
+    c = cases.cylinder_flow(target_elems=elems, rorder=10, torder=10)
     prm = cases.base_params(u0=0.6, kn=0.1, length=1.0, rv=5.5, alpha=alpha)
     return cases.Case("cylinder_lobatto10", c.mesh, eta, w, 10, prm, nsteps=10)
 

@@ -179,3 +179,26 @@ def test_graph_replay_equals_eager_launches():
     assert np.array_equal(a.get_f(), b.get_f())
     assert ra == rb
     assert a.itime == b.itime == 260
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_change_residual_is_max_step_difference(precision):
+    """The record's `change` is max |f^{k+1} - f^k| over every node and
+    ordinate after the clamp and the inflow reset: recomputed here from f
+    read back before and after each step (wall and inflow elements
+    included)."""
+    from dgbgk.solver import Solver
+    case = cylinder_small(elems=800)
+    print(f"case change residual precision={precision}", flush=True)
+    s = Solver(case, precision=precision)
+    f0 = s.get_f()
+    for _ in range(3):
+        r = s.step(1)[0]
+        f1 = s.get_f()
+        want = np.max(np.abs(f1 - f0))
+        # FP64: the same subtraction; FP32: rounded once more in float
+        tol = 0.0 if precision == 0 else 2 ** -23 * want
+        assert abs(r["change"] - want) <= tol
+        # the reference residual counts the increment before the reset
+        assert r["sumres"] >= r["change"] * (1 - 1e-6)
+        f0 = f1
