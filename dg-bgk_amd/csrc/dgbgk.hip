@@ -42,6 +42,25 @@ __device__ __forceinline__ int xcd_logical(int bid, int nblocks) {
 __device__ __forceinline__ float dexp(float x) { return __expf(x); }
 __device__ __forceinline__ double dexp(double x) { return exp(x); }
 
+// Maxwellian exponential with beta^2 pre-scaled by bscale<T>(): FP32 takes
+// the hardware base-2 exponential (beta^2 log2(e) folded in once per block),
+// FP64 the correctly rounded exp.
+template <typename T>
+__device__ __forceinline__ T bscale();
+template <>
+__device__ __forceinline__ float bscale<float>() { return 1.4426950408889634f; }
+template <>
+__device__ __forceinline__ double bscale<double>() { return 1.0; }
+__device__ __forceinline__ float mexp(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ double mexp(double x) { return exp(x); }
+
+// Loads from tables the kernel never writes, through the constant address
+// space: a uniform index then becomes one scalar load.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p, int i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 template <typename T>
 __device__ __forceinline__ void edge_rhsi(T ux, T uy, T nx, T ny, T alen, T g1, T g2,
                                           T& r1, T& r2) {
@@ -107,8 +126,8 @@ __device__ __forceinline__ void nsum(double& s, double& c, double x) {
 // What one slot keeps in registers over all ordinates of its block.
 template <typename T>
 struct SlotState {
-  T gx[3], gy[3], pA[3], pB[3], pU[3], pV[3], pM[3];
-  T area, mmat, nx[3], ny[3], al[3];
+  T gx[3], gy[3], pA[3], pB[3], pU[3], pV[3], pMd[3];  // phase A (all slots)
+  T gxa[3], gya[3], mdt, nx[3], ny[3], al[3];          // phase B (owned slots)
   int info[3], mask, wsl, e;
   bool owned, active;
 };
@@ -137,37 +156,41 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
   const int slot = threadIdx.x;
   const size_t N3 = 3 * (size_t)a.E;
   const T c13 = T(1) / T(3);
-  T coll[3] = {}, U = 0;
-  if (s.active) {
-    T dive = 0, cs = 0;
+  // collision term times DTE: pMd = MU * DTE (src/getinc.f90:34-41)
+  // phase A runs on every slot without a branch: slots without an element
+  // hold zero parameters and element 0's f, and nobody reads their result
+  T cold[3], h[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const T dx = s.pU[k] - ux, dy = s.pV[k] - uy;
-      const T feq = s.pA[k] * dexp(-s.pB[k] * (dx * dx + dy * dy));
-      coll[k] = s.pM[k] * (feq - f[k]);
-      dive += s.gx[k] * (ux * f[k]) + s.gy[k] * (uy * f[k]);
-      cs += coll[k];
-    }
-    dive = dive - c13 * cs;
-    U = (dive * dte) * T(0.5);
-    store4(sf + 4 * slot, f[0], f[1], f[2], U);
+  for (int k = 0; k < 3; ++k) {
+    const T dx = s.pU[k] - ux, dy = s.pV[k] - uy;
+    const T feq = s.pA[k] * mexp(-s.pB[k] * (dx * dx + dy * dy));
+    cold[k] = s.pMd[k] * (feq - f[k]);
   }
+  const T cs = cold[0] + cold[1] + cold[2];
+  // GETELC's predictor: U = DTE/2 (div(c f) - (1/3) sum MU (f_eq - f))
+  const T sx = s.gx[0] * f[0] + s.gx[1] * f[1] + s.gx[2] * f[2];
+  const T sy = s.gy[0] * f[0] + s.gy[1] * f[1] + s.gy[2] * f[2];
+  const T U = T(0.5) * ((ux * sx + uy * sy) * dte - c13 * cs);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) h[k] = f[k] - U;
+  store4(sf + 4 * slot, h[0], h[1], h[2], T(0));
   __syncthreads();
   if (!s.owned) return;
-  // volume term (src/getelc.f90:142-172)
-  const T fxme = c13 * (ux * (f[0] - U) + ux * (f[1] - U) + ux * (f[2] - U));
-  const T fyme = c13 * (uy * (f[0] - U) + uy * (f[1] - U) + uy * (f[2] - U));
+  // volume term (src/getelc.f90:142-172): area * grad N_a . c (mean of f - U)
+  const T sm = c13 * (h[0] + h[1] + h[2]);
+  const T fx = ux * sm, fy = uy * sm;
   T rhs[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) rhs[k] = (s.gx[k] * s.area) * fxme + (s.gy[k] * s.area) * fyme;
+  for (int k = 0; k < 3; ++k) rhs[k] = s.gxa[k] * fx + s.gya[k] * fy;
   // edges (src/edgflxopt.f90:72-125, src/getbou.f90:142-187).  Local edge k
   // runs from node k to node k1 = (k+1)%3 with this element's outward
-  // normal.  The reference orients each side once (left element) and
-  // subtracts the left element's flux from the right element; negating the
-  // normal is exact in floating point, so the right element's outward form
-  // gives the same values.  Upwind: own values when c.n > 0 (on a tie the
-  // right element's values, as in the reference), always on a boundary
-  // side; inflow and wall sides have zero length here.
+  // normal n; its flux adds -(len/6) (c.n) M2 (g_k, g_k1) with
+  // M2 = [[2,1],[1,2]] and g = f - U of the upwind element.  The reference
+  // orients each side once (left element) and subtracts the left element's
+  // flux from the right element; with the right element's outward normal
+  // that is the same expression.  Upwind: own values when c.n > 0 (on a
+  // tie the right element's values, as in the reference), always on a
+  // boundary side; inflow and wall sides have zero length here.
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int k1 = (k + 1) % 3;
@@ -178,25 +201,23 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
     const int j1 = own ? k : ((in >> kUpdNbK) & 3);
     const int j2 = own ? k1 : ((in >> kUpdNbK1) & 3);
     const T* q = sf + 4 * src;
-    const T us = q[3];
-    const T g1 = q[j1] - us, g2 = q[j2] - us;
-    T r1, r2;
-    edge_rhsi(ux, uy, s.nx[k], s.ny[k], s.al[k], g1, g2, r1, r2);
-    rhs[k] += r1;
-    rhs[k1] += r2;
+    const T g1 = q[j1], g2 = q[j2];
+    const T w = -(s.al[k] * cn);
+    rhs[k] += w * (T(2) * g1 + g2);
+    rhs[k1] += w * (g1 + T(2) * g2);
   }
   if (s.wsl >= 0) {
     T* wp = a.wpart + ((size_t)s.wsl * a.V + v) * 8;
-    store4(wp, rhs[0], rhs[1], rhs[2], coll[0]);
-    store4(wp + 4, coll[1], coll[2], U, T(0));
+    store4(wp, rhs[0], rhs[1], rhs[2], cold[0]);
+    store4(wp + 4, cold[1], cold[2], U, T(0));
     return;
   }
   // GETINC + ADTHEM + INFLOW (src/getinc.f90:30-42, src/adthem.f90:109-118,
-  // src/inflow.f90:165-181)
+  // src/inflow.f90:165-181); mdt = 6/RJ * DTE
   T* fo = a.fnxt + (size_t)v * N3 + 3 * (size_t)s.e;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const T d = s.mmat * (dte * rhs[k]) + coll[k] * dte;
+    const T d = s.mdt * rhs[k] + cold[k];
     res = fmax(res, fabs(d));
     T fn = f[k] + d;
     fn = fn < T(0) ? T(0) : fn;
@@ -207,10 +228,11 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
 }
 
 // kDepth: ordinates whose f is in flight ahead of the one being computed.
-// The queue is indexed by compile-time slots of an unrolled loop, so no
-// register copy ever waits on a load that is still outstanding.
+// kDepth + 1 register slots are indexed by compile-time positions of an
+// unrolled loop, so no register copy ever waits on a load still in flight.
 template <typename T, int kDepth>
 __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
+  constexpr int kQ = kDepth + 1;
   __shared__ __align__(16) T s_f[2][kSlots * 4];
   __shared__ double s_red[8];
   const int logical = xcd_logical(blockIdx.x, gridDim.x);
@@ -228,13 +250,14 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
   s.active = slot < nown + nh;
   s.e = s.owned ? e0 + slot : (s.active ? a.halo[hoff + slot - nown] : 0);
   const int e = s.e;
+  const T dte = (T)a.scal[0];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    s.gx[k] = s.gy[k] = s.pA[k] = s.pB[k] = s.pU[k] = s.pV[k] = s.pM[k] = T(0);
-    s.nx[k] = s.ny[k] = s.al[k] = T(0);
+    s.gx[k] = s.gy[k] = s.pA[k] = s.pB[k] = s.pU[k] = s.pV[k] = s.pMd[k] = T(0);
+    s.gxa[k] = s.gya[k] = s.nx[k] = s.ny[k] = s.al[k] = T(0);
     s.info[k] = kUpdOwn;
   }
-  s.area = s.mmat = T(0);
+  s.mdt = T(0);
   s.mask = 0;
   s.wsl = -1;
   if (s.active) {
@@ -244,17 +267,19 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
       s.gy[k] = a.geo[(3 + k) * E + e];
       const size_t nd = 3 * (size_t)e + k;
       s.pA[k] = a.np[nd];
-      s.pB[k] = a.np[N3 + nd];
+      s.pB[k] = a.np[N3 + nd] * bscale<T>();
       s.pU[k] = a.np[2 * N3 + nd];
       s.pV[k] = a.np[3 * N3 + nd];
-      s.pM[k] = a.np[4 * N3 + nd];
+      s.pMd[k] = a.np[4 * N3 + nd] * dte;
     }
   }
   if (s.owned) {
-    s.area = a.geo[6 * E + e];
-    s.mmat = a.geo[7 * E + e];
+    const T area = a.geo[6 * E + e];
+    s.mdt = a.geo[7 * E + e] * dte;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
+      s.gxa[k] = s.gx[k] * area;
+      s.gya[k] = s.gy[k] * area;
       s.info[k] = a.uinfo[k * E + e];
       s.nx[k] = a.unx[k * E + e];
       s.ny[k] = a.uny[k * E + e];
@@ -263,49 +288,46 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
     s.mask = a.emask[e];
     s.wsl = a.ewslot[e];
   }
-  const T dte = (T)a.scal[0];
   T res = 0, chg = 0;
 
-  // prime the queue: f and the ordinate's velocity for v0 .. v0+kDepth-1
-  T q[kDepth][3], qx[kDepth], qy[kDepth], qi[kDepth];
+  // prime the queue: f and the ordinate's velocity and inflow value for
+  // v0 .. v0+kDepth-1
+  T q[kQ][3], qx[kQ], qy[kQ], qi[kQ];
   const T* fe = a.fcur + 3 * (size_t)e;
 #pragma unroll
-  for (int j = 0; j < kDepth; ++j) {
-    q[j][0] = q[j][1] = q[j][2] = T(0);
-    qx[j] = qy[j] = qi[j] = T(0);
-    const int v = v0 + j;
-    if (v < v1) {
-      qx[j] = a.vtab[v];
-      qy[j] = a.vtab[V + v];
-      qi[j] = a.vtab[2 * V + v];
-      if (s.active) {
-        const T* fp = fe + (size_t)v * N3;
-        q[j][0] = fp[0];
-        q[j][1] = fp[1];
-        q[j][2] = fp[2];
-      }
+  for (int j = 0; j < kQ; ++j) {
+    q[j][0] = q[j][1] = q[j][2] = qx[j] = qy[j] = qi[j] = T(0);
+    if (j < kDepth) {  // slots without an element read element 0
+      const int v = min(v0 + j, v1 - 1);
+      const T* fp = fe + (size_t)v * N3;
+      q[j][0] = fp[0];
+      q[j][1] = fp[1];
+      q[j][2] = fp[2];
+      qx[j] = ldc(a.vtab, v);
+      qy[j] = ldc(a.vtab, V + v);
+      qi[j] = ldc(a.vtab, 2 * V + v);
     }
   }
-  for (int vb = v0; vb < v1; vb += kDepth) {
+  for (int vb = v0; vb < v1; vb += kQ) {
 #pragma unroll
-    for (int j = 0; j < kDepth; ++j) {
+    for (int j = 0; j < kQ; ++j) {
       const int v = vb + j;
       if (v < v1) {  // uniform over the block
-        const T f[3] = {q[j][0], q[j][1], q[j][2]};
-        const T ux = qx[j], uy = qy[j], fin = qi[j];
-        const int vn = v + kDepth;
-        if (vn < v1) {
-          qx[j] = a.vtab[vn];
-          qy[j] = a.vtab[V + vn];
-          qi[j] = a.vtab[2 * V + vn];
-          if (s.active) {
-            const T* fp = fe + (size_t)vn * N3;
-            q[j][0] = fp[0];
-            q[j][1] = fp[1];
-            q[j][2] = fp[2];
-          }
+        // unconditional (clamped at the chunk's last ordinate), so the load
+        // lands in its slot without a branch that would force a wait
+        const int vn = min(v + kDepth, v1 - 1);
+        const int jn = (j + kDepth) % kQ;
+        {
+          const T* fp = fe + (size_t)vn * N3;
+          q[jn][0] = fp[0];
+          q[jn][1] = fp[1];
+          q[jn][2] = fp[2];
+          qx[jn] = ldc(a.vtab, vn);
+          qy[jn] = ldc(a.vtab, V + vn);
+          qi[jn] = ldc(a.vtab, 2 * V + vn);
         }
-        update_ordinate<T>(a, s, v, ux, uy, fin, f, s_f[(v - v0) & 1], dte, res, chg);
+        update_ordinate<T>(a, s, v, qx[j], qy[j], qi[j], q[j], s_f[(v - v0) & 1], dte, res,
+                           chg);
       }
     }
   }
@@ -344,7 +366,8 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
     const size_t N3 = 3 * (size_t)E;
     const int vb = a.ngroups > 1 ? grp * a.T_ : 0;
     const int ve = a.ngroups > 1 ? vb + a.T_ : V;
-    const T mmat = a.geo[7 * E + e];
+    const T dte = (T)a.scal[0];
+    const T mdt = a.geo[7 * E + e] * dte;
     int info[3], nsl[3];
     T enx[3], eny[3], eal[3];
 #pragma unroll
@@ -356,12 +379,11 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
       eal[k] = a.ealen[k * E + e];
     }
     const int mask = a.emask[e];
-    const T dte = (T)a.scal[0];
     const T alpha = a.alpha;
     for (int v = vb; v < ve; ++v) {
       const T* wp = a.wpart + ((size_t)we * V + v) * 8;
       T rhs[3] = {wp[0], wp[1], wp[2]};
-      const T coll[3] = {wp[3], wp[4], wp[5]};
+      const T cold[3] = {wp[3], wp[4], wp[5]};  // collision term times DTE
       const T U = wp[6];
       const T ux = a.vtab[v], uy = a.vtab[V + v];
       const T* fc = a.fcur + (size_t)v * N3 + 3 * (size_t)e;
@@ -408,7 +430,7 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
       const T fin = a.vtab[2 * V + v];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const T d = mmat * (dte * rhs[k]) + coll[k] * dte;
+        const T d = mdt * rhs[k] + cold[k];
         res = fmax(res, (double)fabs(d));
         T fn = f[k] + d;
         fn = fn < T(0) ? T(0) : fn;
@@ -446,25 +468,42 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
     const double* cxb = a.vtd + 2 * V;
     const double* cyb = a.vtd + 3 * V;
     const double* wv = a.vtd + 4 * V;
+    // FP64 form: plain sums in ordinate order, as the reference.  FP32-bulk
+    // form: FP64 sums of each run of four ordinates, added with Kahan
+    // compensation.
     double s0 = 0, c0 = 0, s1 = 0, c1 = 0, s2 = 0, c2 = 0;
     int v = 0;
     for (; v + 4 <= V; v += 4) {
       double fv[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) fv[j] = (double)fp[(size_t)(v + j) * N3];
+      if (kComp) {
+        double b0 = 0, b1 = 0, b2 = 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const double fw = fv[j] * wv[v + j];
-        nsum<kComp>(s0, c0, fw);
-        nsum<kComp>(s1, c1, cxb[v + j] * fw);
-        nsum<kComp>(s2, c2, cyb[v + j] * fw);
+        for (int j = 0; j < 4; ++j) {
+          const double fw = fv[j] * ldc(wv, v + j);
+          b0 += fw;
+          b1 += ldc(cxb, v + j) * fw;
+          b2 += ldc(cyb, v + j) * fw;
+        }
+        nsum<true>(s0, c0, b0);
+        nsum<true>(s1, c1, b1);
+        nsum<true>(s2, c2, b2);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double fw = fv[j] * ldc(wv, v + j);
+          s0 += fw;
+          s1 += ldc(cxb, v + j) * fw;
+          s2 += ldc(cyb, v + j) * fw;
+        }
       }
     }
     for (; v < V; ++v) {
-      const double fw = (double)fp[(size_t)v * N3] * wv[v];
+      const double fw = (double)fp[(size_t)v * N3] * ldc(wv, v);
       nsum<kComp>(s0, c0, fw);
-      nsum<kComp>(s1, c1, cxb[v] * fw);
-      nsum<kComp>(s2, c2, cyb[v] * fw);
+      nsum<kComp>(s1, c1, ldc(cxb, v) * fw);
+      nsum<kComp>(s2, c2, ldc(cyb, v) * fw);
     }
     const double n = s0 - c0;
     double nd = 0.0, ux = 0.0, uy = 0.0;
@@ -480,24 +519,37 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
       double fv[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) fv[j] = (double)fp[(size_t)(v + j) * N3];
+      double bp = 0, bt = 0;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const double fw = fv[j] * wv[v + j];
-        const double dx = cxb[v + j] - ux, dy = cyb[v + j] - uy;
-        nsum<kComp>(sp, cp, 0.5 * (dx * dx + dy * dy) * fw);
-        if (dens) {
-          const double ex = ux - cxa[v + j], ey = uy - cya[v + j];
-          nsum<kComp>(st, ct, sqrt(ex * ex + ey * ey) * fw);
+        const double fw = fv[j] * ldc(wv, v + j);
+        const double dx = ldc(cxb, v + j) - ux, dy = ldc(cyb, v + j) - uy;
+        const double ex = ux - ldc(cxa, v + j), ey = uy - ldc(cya, v + j);
+        const double e2 = ex * ex + ey * ey;
+        // FP32-bulk: the hardware square root (single-precision accuracy,
+        // far below the FP32 rounding of f); FP64: correctly rounded
+        const double rel = kComp ? __builtin_amdgcn_sqrt(e2) : sqrt(e2);
+        if (kComp) {
+          bp += 0.5 * (dx * dx + dy * dy) * fw;
+          bt += rel * fw;
+        } else {
+          sp += 0.5 * (dx * dx + dy * dy) * fw;
+          if (dens) st += rel * fw;
         }
+      }
+      if (kComp) {
+        nsum<true>(sp, cp, bp);
+        if (dens) nsum<true>(st, ct, bt);
       }
     }
     for (; v < V; ++v) {
-      const double fw = (double)fp[(size_t)v * N3] * wv[v];
-      const double dx = cxb[v] - ux, dy = cyb[v] - uy;
+      const double fw = (double)fp[(size_t)v * N3] * ldc(wv, v);
+      const double dx = ldc(cxb, v) - ux, dy = ldc(cyb, v) - uy;
       nsum<kComp>(sp, cp, 0.5 * (dx * dx + dy * dy) * fw);
       if (dens) {
-        const double ex = ux - cxa[v], ey = uy - cya[v];
-        nsum<kComp>(st, ct, sqrt(ex * ex + ey * ey) * fw);
+        const double ex = ux - ldc(cxa, v), ey = uy - ldc(cya, v);
+        const double e2 = ex * ex + ey * ey;
+        nsum<kComp>(st, ct, (kComp ? __builtin_amdgcn_sqrt(e2) : sqrt(e2)) * fw);
       }
     }
     const double ps = (sp - cp) * a.mm;
@@ -761,7 +813,7 @@ struct Ctx {
   StepRec* rec = nullptr;
   int rec_cap = 0;
   int wall_tpb = 256;
-  int depth = 1;  // update kernel: ordinates of f in flight (1..4)
+  int depth = 2;  // update kernel: ordinates of f in flight (1 or 2)
   int ntiles = 0, vchunk = 1, nvchunks = 1, nblk_upd = 0, nblk_wall = 0, ngroups = 1,
       nblk_mom = 0, neta = 0, nforce = 0;
   int itime = 0;
@@ -852,12 +904,10 @@ void tend(Ctx& c, int k) {
 template <typename T>
 void launch_update(Ctx& c, const UpdArgs<T>& u) {
   const dim3 g(c.nblk_upd), b(kSlots);
-  switch (c.depth) {
-    case 1: hipLaunchKernelGGL((k_update<T, 1>), g, b, 0, c.stream, u); break;
-    case 2: hipLaunchKernelGGL((k_update<T, 2>), g, b, 0, c.stream, u); break;
-    case 3: hipLaunchKernelGGL((k_update<T, 3>), g, b, 0, c.stream, u); break;
-    default: hipLaunchKernelGGL((k_update<T, 4>), g, b, 0, c.stream, u); break;
-  }
+  if (c.depth == 2)
+    hipLaunchKernelGGL((k_update<T, 2>), g, b, 0, c.stream, u);
+  else
+    hipLaunchKernelGGL((k_update<T, 1>), g, b, 0, c.stream, u);
 }
 
 template <typename T>
@@ -1217,13 +1267,13 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   c.gs = prm->order == DGBGK_ORDER_REFERENCE ? 1 : 0;
   if (const char* ng = std::getenv("DGBGK_NO_GRAPH")) c.use_graph = ng[0] == '0';
   c.tsz = c.prec ? 4 : 8;
-  // ordinates of f in flight per update thread beyond the one computed:
-  // one is fastest on gfx950 (deeper queues cost more in occupancy than
-  // they gain in latency hiding)
-  c.depth = 1;
+  // ordinates of f in flight per update thread beyond the one computed
+  // (two measured marginally ahead of one on gfx950; deeper queues cost
+  // more in occupancy than they gain)
+  c.depth = 2;
   if (const char* pd = std::getenv("DGBGK_PREFETCH")) {
     const int d = std::atoi(pd);
-    if (d >= 1 && d <= 4) c.depth = d;
+    if (d >= 1 && d <= 2) c.depth = d;
   }
   const Prep& p = c.p;
   auto cleanup = [&](int code) {
