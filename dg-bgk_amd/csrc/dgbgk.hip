@@ -149,10 +149,13 @@ __device__ __forceinline__ void store4<double>(double* p, double a, double b, do
 // this ordinate's LDS buffer, [slot][4] = f at the three nodes and U.
 // Contains the block's one barrier per ordinate, so every thread of the
 // block calls it for every v.
-template <typename T>
+template <typename T, bool kMom>
 __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotState<T>& s,
                                                 int v, T ux, T uy, T fin, const T (&f)[3],
-                                                T* __restrict__ sf, T dte, T& res, T& chg) {
+                                                T* __restrict__ sf, T dte, T& res, T& chg,
+                                                double wv, double cx, double cy,
+                                                double (&m0)[3], double (&m1)[3],
+                                                double (&m2)[3]) {
   const int slot = threadIdx.x;
   const size_t N3 = 3 * (size_t)a.E;
   const T c13 = T(1) / T(3);
@@ -224,16 +227,27 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
     if ((s.mask >> k) & 1) fn = fin;
     chg = fmax(chg, fabs(fn - f[k]));
     fo[k] = fn;
+    if (kMom) {
+      // MACROS' first pass (src/macros.f90:93-108) on f^{k+1}, this chunk's
+      // ordinates, in FP64
+      const double fw = (double)fn * wv;
+      m0[k] += fw;
+      m1[k] += cx * fw;
+      m2[k] += cy * fw;
+    }
   }
 }
 
 // kDepth: ordinates whose f is in flight ahead of the one being computed.
 // kDepth + 1 register slots are indexed by compile-time positions of an
 // unrolled loop, so no register copy ever waits on a load still in flight.
-template <typename T, int kDepth>
-__global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
+// kMom: also accumulate the first moment pass of f^{k+1} (n, n u, n v per
+// node over the block's ordinates) into msum.
+// kS: slots (threads) per block, the tile size the host built for.
+template <typename T, int kDepth, bool kMom, int kS>
+__global__ void __launch_bounds__(kS) k_update(UpdArgs<T> a) {
   constexpr int kQ = kDepth + 1;
-  __shared__ __align__(16) T s_f[2][kSlots * 4];
+  __shared__ __align__(16) T s_f[2][kS * 4];
   __shared__ double s_red[8];
   const int logical = xcd_logical(blockIdx.x, gridDim.x);
   const int tile = logical % a.ntiles;
@@ -293,10 +307,12 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
   // prime the queue: f and the ordinate's velocity and inflow value for
   // v0 .. v0+kDepth-1
   T q[kQ][3], qx[kQ], qy[kQ], qi[kQ];
+  double qw[kQ], qcx[kQ], qcy[kQ];
   const T* fe = a.fcur + 3 * (size_t)e;
 #pragma unroll
   for (int j = 0; j < kQ; ++j) {
     q[j][0] = q[j][1] = q[j][2] = qx[j] = qy[j] = qi[j] = T(0);
+    qw[j] = qcx[j] = qcy[j] = 0.0;
     if (j < kDepth) {  // slots without an element read element 0
       const int v = min(v0 + j, v1 - 1);
       const T* fp = fe + (size_t)v * N3;
@@ -306,8 +322,14 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
       qx[j] = ldc(a.vtab, v);
       qy[j] = ldc(a.vtab, V + v);
       qi[j] = ldc(a.vtab, 2 * V + v);
+      if (kMom) {
+        qw[j] = ldc(a.vtd, 4 * V + v);
+        qcx[j] = ldc(a.vtd, 2 * V + v);
+        qcy[j] = ldc(a.vtd, 3 * V + v);
+      }
     }
   }
+  double m0[3] = {0, 0, 0}, m1[3] = {0, 0, 0}, m2[3] = {0, 0, 0};
   for (int vb = v0; vb < v1; vb += kQ) {
 #pragma unroll
     for (int j = 0; j < kQ; ++j) {
@@ -325,10 +347,24 @@ __global__ void __launch_bounds__(kSlots) k_update(UpdArgs<T> a) {
           qx[jn] = ldc(a.vtab, vn);
           qy[jn] = ldc(a.vtab, V + vn);
           qi[jn] = ldc(a.vtab, 2 * V + vn);
+          if (kMom) {
+            qw[jn] = ldc(a.vtd, 4 * V + vn);
+            qcx[jn] = ldc(a.vtd, 2 * V + vn);
+            qcy[jn] = ldc(a.vtd, 3 * V + vn);
+          }
         }
-        update_ordinate<T>(a, s, v, qx[j], qy[j], qi[j], q[j], s_f[(v - v0) & 1], dte, res,
-                           chg);
+        update_ordinate<T, kMom>(a, s, v, qx[j], qy[j], qi[j], q[j], s_f[(v - v0) & 1], dte,
+                                 res, chg, qw[j], qcx[j], qcy[j], m0, m1, m2);
       }
+    }
+  }
+  if (kMom && s.owned && s.wsl < 0) {
+    double* ms = a.msum + (size_t)(logical / a.ntiles) * 3 * N3 + 3 * (size_t)e;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      ms[k] = m0[k];
+      ms[N3 + k] = m1[k];
+      ms[2 * N3 + k] = m2[k];
     }
   }
   const double r = block_max((double)res, s_red);
@@ -356,7 +392,7 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
   // ring): the in-place reads of ordinates < v come from LDS
   extern __shared__ unsigned char s_dyn[];
   T* s_ring = reinterpret_cast<T*>(s_dyn) + (size_t)threadIdx.x * 3 * a.T_;
-  const bool use_lds = a.ngroups > 1 && !a.jacobi;
+  const bool use_lds = a.lds_ring;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   double res = 0.0, chg = 0.0;
   if (gid < a.nwe * a.ngroups) {
@@ -364,8 +400,8 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
     const int e = a.wall_elems[we];
     const int E = a.E, V = a.V;
     const size_t N3 = 3 * (size_t)E;
-    const int vb = a.ngroups > 1 ? grp * a.T_ : 0;
-    const int ve = a.ngroups > 1 ? vb + a.T_ : V;
+    const int vb = grp * a.gsize;
+    const int ve = vb + a.gsize;
     const T dte = (T)a.scal[0];
     const T mdt = a.geo[7 * E + e] * dte;
     int info[3], nsl[3];
@@ -399,6 +435,10 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
         if (i1 < 0) {  // molecules leaving through the wall: own values
           g1 = f[a1] - U;
           g2 = f[a2] - U;
+        } else if (a.diffuse) {  // alpha = 1: (1 - alpha) * Rf is exactly 0
+          const T mf = a.refl_t[3 * rk + 2] * (T)a.eta[a.wall_bside[ws]];
+          g1 = alpha * mf;
+          g2 = alpha * mf;
         } else {
           const int i2 = a.refl_i[2 * rk + 1];
           const T zd1 = a.refl_t[3 * rk], zd2 = a.refl_t[3 * rk + 1];
@@ -468,42 +508,46 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
     const double* cxb = a.vtd + 2 * V;
     const double* cyb = a.vtd + 3 * V;
     const double* wv = a.vtd + 4 * V;
-    // FP64 form: plain sums in ordinate order, as the reference.  FP32-bulk
-    // form: FP64 sums of each run of four ordinates, added with Kahan
-    // compensation.
+    // First pass (n, n u, n v), summed per ordinate chunk of the update
+    // kernel and then over the chunks: plainly in the FP64 form (with one
+    // chunk that is the reference's ordinate order), with Kahan compensation
+    // over the chunk sums in the FP32-bulk form.  Inside a step k_update
+    // has left the chunk sums in msum (wall elements excepted), computed
+    // in exactly this order.
     double s0 = 0, c0 = 0, s1 = 0, c1 = 0, s2 = 0, c2 = 0;
-    int v = 0;
-    for (; v + 4 <= V; v += 4) {
-      double fv[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) fv[j] = (double)fp[(size_t)(v + j) * N3];
-      if (kComp) {
-        double b0 = 0, b1 = 0, b2 = 0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const double fw = fv[j] * ldc(wv, v + j);
-          b0 += fw;
-          b1 += ldc(cxb, v + j) * fw;
-          b2 += ldc(cyb, v + j) * fw;
-        }
-        nsum<true>(s0, c0, b0);
-        nsum<true>(s1, c1, b1);
-        nsum<true>(s2, c2, b2);
+    const bool have = a.msum != nullptr && a.ewslot[k / 3] < 0;
+    for (int ch = 0; ch < a.nchunks; ++ch) {
+      double b0 = 0, b1 = 0, b2 = 0;
+      if (have) {
+        const double* ms = a.msum + (size_t)ch * 3 * N3 + k;
+        b0 = ms[0];
+        b1 = ms[N3];
+        b2 = ms[2 * N3];
       } else {
+        const int vb = ch * a.vchunk, ve = min(V, vb + a.vchunk);
+        int v = vb;
+        for (; v + 4 <= ve; v += 4) {
+          double fv[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const double fw = fv[j] * ldc(wv, v + j);
-          s0 += fw;
-          s1 += ldc(cxb, v + j) * fw;
-          s2 += ldc(cyb, v + j) * fw;
+          for (int j = 0; j < 4; ++j) fv[j] = (double)fp[(size_t)(v + j) * N3];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const double fw = fv[j] * ldc(wv, v + j);
+            b0 += fw;
+            b1 += ldc(cxb, v + j) * fw;
+            b2 += ldc(cyb, v + j) * fw;
+          }
+        }
+        for (; v < ve; ++v) {
+          const double fw = (double)fp[(size_t)v * N3] * ldc(wv, v);
+          b0 += fw;
+          b1 += ldc(cxb, v) * fw;
+          b2 += ldc(cyb, v) * fw;
         }
       }
-    }
-    for (; v < V; ++v) {
-      const double fw = (double)fp[(size_t)v * N3] * ldc(wv, v);
-      nsum<kComp>(s0, c0, fw);
-      nsum<kComp>(s1, c1, ldc(cxb, v) * fw);
-      nsum<kComp>(s2, c2, ldc(cyb, v) * fw);
+      nsum<kComp>(s0, c0, b0);
+      nsum<kComp>(s1, c1, b1);
+      nsum<kComp>(s2, c2, b2);
     }
     const double n = s0 - c0;
     double nd = 0.0, ux = 0.0, uy = 0.0;
@@ -514,7 +558,7 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
     }
     double sp = 0, cp = 0, st = 0, ct = 0;
     const bool dens = n >= 1e-20;
-    v = 0;
+    int v = 0;
     for (; v + 4 <= V; v += 4) {
       double fv[4];
 #pragma unroll
@@ -809,11 +853,15 @@ struct Ctx {
       *pt_node = nullptr, *intma = nullptr;
   double *vtd = nullptr, *eta_geo = nullptr, *eta = nullptr, *rj = nullptr, *mom = nullptr,
          *pts = nullptr, *ps_glob = nullptr, *coord = nullptr, *res_part = nullptr,
-         *mu_part = nullptr, *mass_part = nullptr, *scal = nullptr, *ps4 = nullptr;
+         *mu_part = nullptr, *mass_part = nullptr, *scal = nullptr, *ps4 = nullptr,
+         *msum = nullptr;
   StepRec* rec = nullptr;
   int rec_cap = 0;
   int wall_tpb = 256;
-  int depth = 2;  // update kernel: ordinates of f in flight (1 or 2)
+  int gsize = 1, wall_diffuse = 0, wall_lds = 0;  // k_wall ordinates per thread, modes
+  int slots = 256;   // update kernel: tile slots (threads per block)
+  int depth = 1;     // update kernel: ordinates of f in flight (1 or 2)
+  int fuse_mom = 1;  // update kernel also does the first moment pass
   int ntiles = 0, vchunk = 1, nvchunks = 1, nblk_upd = 0, nblk_wall = 0, ngroups = 1,
       nblk_mom = 0, neta = 0, nforce = 0;
   int itime = 0;
@@ -833,8 +881,9 @@ struct Ctx {
 
 namespace {
 
+// partials: k_update of this step left the first moment pass in msum
 template <typename T>
-int launch_moments(Ctx& c) {
+int launch_moments(Ctx& c, bool partials = false) {
   MomArgs<T> m;
   m.f = (const T*)c.f[c.cur];
   m.vtd = c.vtd;
@@ -845,6 +894,10 @@ int launch_moments(Ctx& c) {
   m.mass_part = c.mass_part;
   m.mm = c.p.prm.m / kNa;
   m.sigt = kPi * c.p.prm.d * c.p.prm.d;
+  m.msum = partials ? c.msum : nullptr;
+  m.ewslot = c.ewslot;
+  m.nchunks = c.nvchunks;
+  m.vchunk = c.vchunk;
   m.E = c.p.E;
   m.V = c.p.V;
   hipLaunchKernelGGL(k_moments<T>, dim3(c.nblk_mom), dim3(256), 0, c.stream, m);
@@ -901,13 +954,28 @@ void tend(Ctx& c, int k) {
   c.ev_used += 1;
 }
 
+template <typename T, int kS>
+void launch_update_s(Ctx& c, const UpdArgs<T>& u) {
+  const dim3 g(c.nblk_upd), b(kS);
+  if (c.fuse_mom) {
+    if (c.depth == 2)
+      hipLaunchKernelGGL((k_update<T, 2, true, kS>), g, b, 0, c.stream, u);
+    else
+      hipLaunchKernelGGL((k_update<T, 1, true, kS>), g, b, 0, c.stream, u);
+  } else {
+    if (c.depth == 2)
+      hipLaunchKernelGGL((k_update<T, 2, false, kS>), g, b, 0, c.stream, u);
+    else
+      hipLaunchKernelGGL((k_update<T, 1, false, kS>), g, b, 0, c.stream, u);
+  }
+}
+
 template <typename T>
 void launch_update(Ctx& c, const UpdArgs<T>& u) {
-  const dim3 g(c.nblk_upd), b(kSlots);
-  if (c.depth == 2)
-    hipLaunchKernelGGL((k_update<T, 2>), g, b, 0, c.stream, u);
+  if (c.slots == 512)
+    launch_update_s<T, 512>(c, u);
   else
-    hipLaunchKernelGGL((k_update<T, 1>), g, b, 0, c.stream, u);
+    launch_update_s<T, 256>(c, u);
 }
 
 template <typename T>
@@ -926,7 +994,7 @@ int step_once(Ctx& c) {
   u.fcur = fc; u.fnxt = fn; u.geo = (const T*)c.geo; u.uinfo = c.uinfo;
   u.unx = (const T*)c.unx; u.uny = (const T*)c.uny; u.ual = (const T*)c.ual;
   u.emask = c.emask; u.ewslot = c.ewslot; u.np = (const T*)c.np; u.vtab = (const T*)c.vtab;
-  u.tiles = c.tiles; u.halo = c.halo; u.wpart = (T*)c.wpart;
+  u.tiles = c.tiles; u.halo = c.halo; u.wpart = (T*)c.wpart; u.vtd = c.vtd; u.msum = c.msum;
   u.scal = c.scal; u.res_part = c.res_part; u.res_stride = c.nblk_upd + c.nblk_wall;
   u.E = p.E; u.V = p.V; u.ntiles = c.ntiles; u.vchunk = c.vchunk;
   tbegin(c, 0);
@@ -942,17 +1010,19 @@ int step_once(Ctx& c) {
     w.wall_elems = c.wall_elems; w.wpart = (const T*)c.wpart; w.scal = c.scal;
     w.res_part = c.res_part + c.nblk_upd; w.res_stride = c.nblk_upd + c.nblk_wall;
     w.alpha = (T)p.prm.alpha; w.E = p.E; w.V = p.V;
-    w.T_ = p.T; w.ngroups = c.ngroups; w.nwe = (int)p.wall_elems.size();
+    w.T_ = p.T; w.ngroups = c.ngroups; w.gsize = c.gsize; w.nwe = (int)p.wall_elems.size();
     w.jacobi = c.gs ? 0 : 1;
+    w.diffuse = c.wall_diffuse;
+    w.lds_ring = c.wall_lds;
     tbegin(c, 1);
-    const size_t lds = c.ngroups > 1 ? (size_t)c.wall_tpb * 3 * p.T * sizeof(T) : 0;
+    const size_t lds = c.wall_lds ? (size_t)c.wall_tpb * 3 * p.T * sizeof(T) : 0;
     hipLaunchKernelGGL(k_wall<T>, dim3(c.nblk_wall), dim3(c.wall_tpb), lds, c.stream, w);
     HIPCK(hipGetLastError());
     tend(c, 1);
   }
   c.cur = 1 - c.cur;
   tbegin(c, 2);
-  int st = launch_moments<T>(c);
+  int st = launch_moments<T>(c, c.fuse_mom != 0);
   if (st) return st;
   tend(c, 2);
   tbegin(c, 5);
@@ -1257,8 +1327,12 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   if (!h) return fail(kErrNoMem, "out of host memory");
   Ctx& c = h->c;
   std::string msg;
+  // update tile size: 256 slots; 512 (half the halo per owned element)
+  // on request
+  c.slots = kSlots;
+  if (const char* ps = std::getenv("DGBGK_SLOTS")) c.slots = std::atoi(ps) == 512 ? 512 : 256;
   int st = prepare(nelem, npoin, nboun, intma, coord, bsido, rorder, torder, eta, w,
-                   to_params(prm), kSlots, c.p, msg);
+                   to_params(prm), c.slots, c.p, msg);
   if (st) {
     delete h;
     return fail(st, msg);
@@ -1268,13 +1342,14 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   if (const char* ng = std::getenv("DGBGK_NO_GRAPH")) c.use_graph = ng[0] == '0';
   c.tsz = c.prec ? 4 : 8;
   // ordinates of f in flight per update thread beyond the one computed
-  // (two measured marginally ahead of one on gfx950; deeper queues cost
-  // more in occupancy than they gain)
-  c.depth = 2;
+  // (deeper queues cost more in occupancy than they gain on gfx950), and
+  // whether the update kernel also sums the first moment pass
+  c.depth = 1;
   if (const char* pd = std::getenv("DGBGK_PREFETCH")) {
     const int d = std::atoi(pd);
     if (d >= 1 && d <= 2) c.depth = d;
   }
+  if (const char* fm = std::getenv("DGBGK_FUSE_MOMENTS")) c.fuse_mom = fm[0] != '0';
   const Prep& p = c.p;
   auto cleanup = [&](int code) {
     for (void* q : c.owned) hipFree(q);
@@ -1298,14 +1373,28 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   c.vchunk = std::max(1, std::min(vch, p.V));
   c.nvchunks = (p.V + c.vchunk - 1) / c.vchunk;
   c.nblk_upd = c.ntiles * c.nvchunks;
-  c.ngroups = p.ring_separable ? p.R : 1;
   const int nwe = (int)p.wall_elems.size();
-  // k_wall: threads per block so each thread's ring fits in <= 64 KB of LDS
+  // k_wall: ordinates that depend on each other through in-place reflected
+  // reads are walked by one thread in order.  Without in-place reads
+  // (Jacobi order, or a fully diffuse wall, whose reflected part has weight
+  // 1 - alpha = 0) every (wall element, ordinate) is independent.
   c.wall_tpb = 256;
-  if (c.ngroups > 1) {
+  c.wall_diffuse = prm->alpha == 1.0 ? 1 : 0;
+  c.wall_lds = 0;
+  if (!c.gs || c.wall_diffuse) {
+    c.ngroups = p.V;
+    c.gsize = 1;
+  } else if (p.ring_separable) {
+    // one thread per speed ring; its freshly advanced f is staged in LDS
+    c.ngroups = p.R;
+    c.gsize = p.T;
     const size_t per = 3 * (size_t)p.T * c.tsz;
     while (c.wall_tpb > 64 && (size_t)c.wall_tpb * per > 65536) c.wall_tpb >>= 1;
-    if ((size_t)c.wall_tpb * per > 65536) c.ngroups = 1;  // ring too long: no LDS staging
+    if ((size_t)c.wall_tpb * per <= 65536) c.wall_lds = 1;
+    else c.wall_tpb = 256;
+  } else {
+    c.ngroups = 1;
+    c.gsize = p.V;
   }
   c.nblk_wall = nwe ? (nwe * c.ngroups + c.wall_tpb - 1) / c.wall_tpb : 0;
   c.nblk_mom = (int)((3 * (size_t)p.E + 255) / 256);
@@ -1314,6 +1403,8 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   st = c.prec ? upload_tables<float>(c) : upload_tables<double>(c);
   if (st) return cleanup(st);
   if ((st = alloc(&c.res_part, 2 * ((size_t)c.nblk_upd + c.nblk_wall), c.owned)))
+    return cleanup(st);
+  if ((st = alloc(&c.msum, (size_t)c.nvchunks * 3 * 3 * (size_t)p.E, c.owned)))
     return cleanup(st);
   if ((st = alloc(&c.mu_part, (size_t)c.nblk_mom, c.owned))) return cleanup(st);
   if ((st = alloc(&c.mass_part, (size_t)c.nblk_mom, c.owned))) return cleanup(st);

@@ -54,6 +54,8 @@ struct UpdArgs {
   const int* __restrict__ tiles;   // [ntiles][4] e0, nown, hoff, nhalo
   const int* __restrict__ halo;
   T* __restrict__ wpart;           // [NWE][V][8] partial RHS (3), COLL (3), U
+  const double* __restrict__ vtd;  // [5][V]: cxa, cya, cxb, cyb, wv
+  double* __restrict__ msum;       // [nvchunks][3][3E] moment partials of f^{k+1}
   const double* __restrict__ scal; // [0] = DTE
   double* __restrict__ res_part;   // per block: max|increment|; +res_stride: max|change|
   int res_stride;
@@ -82,8 +84,10 @@ struct WallArgs {
   double* __restrict__ res_part;
   int res_stride;
   T alpha;
-  int E, V, T_, ngroups, nwe;
-  int jacobi;  // reflections read f of the previous step only
+  int E, V, T_, ngroups, gsize, nwe;  // ngroups threads per wall element, gsize ordinates each
+  int jacobi;    // reflections read f of the previous step only
+  int diffuse;   // alpha = 1: the reflected part has weight 0 and is not read
+  int lds_ring;  // one thread per speed ring, in-place reads staged in LDS
 };
 
 template <typename T>
@@ -95,8 +99,10 @@ struct MomArgs {
   T* __restrict__ np;              // [5][3E]
   double* __restrict__ mu_part;
   double* __restrict__ mass_part;
+  const double* __restrict__ msum;  // k_update's partials, or null: two passes
+  const int* __restrict__ ewslot;   // wall elements have no partials
   double mm, sigt;
-  int E, V;
+  int E, V, nchunks, vchunk;
 };
 
 }  // namespace dgbgk

@@ -167,10 +167,79 @@ def results_block(itime: int, a, b, c) -> str:
     return "\n".join(out) + "\n"
 
 
+def _ordinate_xy(eta, torder: int, rv: float):
+    """Cartesian velocity of every ordinate, ring-major, as the reference's
+    restart writer maps VCORD back (src/output.f90:112-118, PI = 3.1416)."""
+    eta = np.asarray(eta, float)
+    iz = np.arange(1, torder + 1)
+    rt = np.repeat(eta * (rv / 2) + rv / 2, torder)
+    th = np.tile(-1.0 + (2.0 * iz) / torder, len(eta)) * 3.1416
+    return rt * np.cos(th), rt * np.sin(th)
+
+
+def write_restart(path: str, f, npoin: int, eta, torder: int, rv: float):
+    """f (E, V, 3) in the reference writer's layout (src/output.f90:54-58,
+    110-125): 'NPOIN     VNPNT', the two counts, then for every ordinate
+    and element 'IE CX CY f1 f2 f3', with f at full double precision."""
+    f = np.asarray(f, float)
+    E, V, _ = f.shape
+    cx, cy = _ordinate_xy(eta, torder, rv)
+    with open(path, "w") as fh:
+        fh.write("NPOIN     VNPNT\n")
+        fh.write(f" {npoin}     {V}\n")
+        ie = np.arange(1, E + 1)
+        for v in range(V):
+            rows = np.column_stack([ie, np.full(E, cx[v]), np.full(E, cy[v]), f[:, v, :]])
+            np.savetxt(fh, rows, fmt=["%10d", "%15.7E", "%15.7E", "%25.17E", "%25.17E",
+                                      "%25.17E"], delimiter=" ")
+
+
+def read_restart(path: str, nelem: int, nvel: int, npoin: int) -> np.ndarray:
+    """f (E, V, 3) from a restart file of write_restart's layout: the last
+    three numbers of each line are the element's three nodal values."""
+    with open(path) as fh:
+        fh.readline()
+        counts = [int(_fnum(t)) for t in fh.readline().split()]
+        if counts != [npoin, nvel]:
+            raise ValueError(f"restart file is for NPOIN, VNPNT = {counts}, "
+                             f"not {[npoin, nvel]}")
+        data = np.loadtxt(fh, converters=None, dtype=float)
+    data = data.reshape(nvel, nelem, -1)
+    return np.ascontiguousarray(data[:, :, -3:].transpose(1, 0, 2))
+
+
+def write_gid_mesh(path: str, mesh: Mesh):
+    """The reference's GiD mesh file (src/output.f90:130-153, formats
+    200-240)."""
+    with open(path, "w") as fh:
+        fh.write("mesh dimension = 2 elemtype triangle nnode = 3\n")
+        fh.write("   COORDINATES\n")
+        for i, (x, y) in enumerate(mesh.coord, 1):
+            fh.write(f"{i:10d}  {x:10.5f}  {y:10.5f}\n")
+        fh.write("end coordinates\n")
+        fh.write("elements\n")
+        for i, t in enumerate(mesh.intma, 1):
+            fh.write(f"{i:8d}{t[0]:8d}{t[1]:8d}{t[2]:8d}\n")
+        fh.write("end elements\n")
+
+
+def read_gid_mesh(path: str):
+    """(coord (P, 2), intma (E, 3)) from a GiD mesh file."""
+    lines = open(path).read().splitlines()
+    a = lines.index("   COORDINATES") + 1
+    b = lines.index("end coordinates")
+    c = lines.index("elements") + 1
+    d = lines.index("end elements")
+    coord = np.array([[float(t) for t in l.split()[1:3]] for l in lines[a:b]])
+    intma = np.array([[int(t) for t in l.split()[1:4]] for l in lines[c:d]], dtype=np.int32)
+    return coord, intma
+
+
 def write_run_dir(case, directory: str, ntime: int, forceout: int = 1,
-                  deck_name: str = "run.con") -> str:
+                  deck_name: str = "run.con", restart_in: str | None = None) -> str:
     """Writes deck + mesh + velocity file of a Case as a reference-style run
-    directory; returns the deck path."""
+    directory; returns the deck path.  restart_in: start from that restart
+    file (IVD%RS = 1) instead of the INICON2 state."""
     os.makedirs(directory, exist_ok=True)
     write_plt(os.path.join(directory, "mesh.plt"), case.mesh)
     write_lobatto(os.path.join(directory, "velocity.txt"), case.eta, case.w)
@@ -178,13 +247,16 @@ def write_run_dir(case, directory: str, ntime: int, forceout: int = 1,
     keys = {
         "TORDER": int(case.torder), "rv": p["rv"], "FORCEOUT": int(forceout),
         "NTIME": int(ntime), "IMMAT": 1, "ALPHA": p["alpha"], "CSAFM": p["csafm"],
-        "RS": 0, "T1": p["t1"], "P1": p["p1"], "U0": p["u0"], "V0": p["v0"],
+        "RS": 1 if restart_in else 0, "T1": p["t1"], "P1": p["p1"], "U0": p["u0"], "V0": p["v0"],
         "INF": int(p.get("inf", 1)), "CINF": list(p["cinf"]), "NVSPACEPART": 1,
         "R": p["rgas"], "d": p["d"], "M": p["m"],
         "LobattoFile": "velocity.txt", "PSpaceFile": "mesh.plt", "OutFile": "FILE.OUT",
         "ResidualFile": "RESIDUAL.RES", "ResultsFile1": "RESULTS1.RES",
-        "ResultsFile2": "RESULTS2.RES",
+        "ResultsFile2": "RESULTS2.RES", "RestartOutFile": "RESTART_OUT.RES",
+        "GIDMeshFile": "GIDMESH.RES",
     }
+    if restart_in:
+        keys["RestartInFile"] = restart_in
     path = os.path.join(directory, deck_name)
     write_deck(path, keys)
     return path

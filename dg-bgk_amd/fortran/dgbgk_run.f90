@@ -1,7 +1,7 @@
 ! dgbgk_run.f90 -- Fortran 90 host for the DG-BGK time march on MI355X.
 !
 ! Usage:  dgbgk_run <deck.con> [--fp32] [--jacobi] [--lookup=reference|corrected]
-!                              [--check]
+!                              [--check] [--restart-out]
 !
 ! Reads the reference's run deck (namelist /Input/ IVD, the same keys and
 ! defaults as src/spectralbgk.f90:116-148), its velocity file
@@ -11,6 +11,16 @@
 ! (src/advnce.f90:103-120,163-170,403-435,468-489).  The time march itself
 ! runs on the GPU through the C ABI (module DGBGK_API).  --check builds the
 ! host-side setup only and prints its sizes (no GPU needed).
+!
+! Restart: with IVD%RS = 1 the initial f is read from IVD%RestartInFile;
+! --restart-out writes f to IVD%RestartOutFile and the mesh to
+! IVD%GIDMeshFile after the loop.  The layout is the reference writer's
+! (src/output.f90:54-58,110-125: a 'NPOIN     VNPNT' line, the two counts,
+! then per ordinate and element "IE CX CY f1 f2 f3") at full precision,
+! and the reader takes the last three numbers of each line.  (The
+! reference's writer is not called and its reader, src/inicon2.f90:87-110,
+! reads the first three numbers of such a line as f, so the two do not
+! round-trip.)
 PROGRAM DGBGK_RUN
   USE ISO_C_BINDING
   USE DGBGK_API
@@ -37,14 +47,16 @@ PROGRAM DGBGK_RUN
   TYPE(DGBGK_PARAMS_T) :: PRM
   TYPE(DGBGK_STEP_INFO_T), ALLOCATABLE :: INFO(:)
   TYPE(C_PTR) :: CTX, SETUP
-  LOGICAL :: CHECK
+  LOGICAL :: CHECK, RSOUT
   REAL(8) :: T0, T1S, SUMW
+  REAL(C_DOUBLE), ALLOCATABLE :: F(:, :, :)
 
   PRM%PRECISION = DGBGK_FP64
   PRM%ORDER = DGBGK_ORDER_REFERENCE
   PRM%LOOKUP = DGBGK_LOOKUP_CORRECTED
   PRM%VCHUNK = 0
   CHECK = .FALSE.
+  RSOUT = .FALSE.
   NARG = COMMAND_ARGUMENT_COUNT()
   IF (NARG .LT. 1) THEN
     WRITE (*, *) 'usage: dgbgk_run <deck.con> [--fp32] [--jacobi] [--lookup=reference]'
@@ -64,6 +76,8 @@ PROGRAM DGBGK_RUN
       PRM%LOOKUP = DGBGK_LOOKUP_CORRECTED
     CASE ('--check')
       CHECK = .TRUE.
+    CASE ('--restart-out')
+      RSOUT = .TRUE.
     CASE DEFAULT
       WRITE (*, *) 'unknown option ', TRIM(ARG)
       STOP 2
@@ -86,10 +100,6 @@ PROGRAM DGBGK_RUN
   OPEN (13, FILE=DECK, STATUS='OLD')
   READ (13, Input)
   CLOSE (13)
-  IF (IVD%RS .NE. 0) THEN
-    WRITE (*, *) 'restart input (RS=1) is not supported by this host'
-    STOP 3
-  ENDIF
 
   ! velocity file: RORDER, then RORDER lines "eta weight"
   OPEN (10, FILE=IVD%LobattoFile, STATUS='OLD')
@@ -168,6 +178,12 @@ PROGRAM DGBGK_RUN
     STOP 1
   ENDIF
   ALLOCATE (ND(NPOIN), RHO(NPOIN), UVEL(NPOIN), VVEL(NPOIN), PS(NPOIN), TEMP(NPOIN))
+  IF (IVD%RS .EQ. 1) THEN
+    ! restart input replaces the INICON2 state (src/inicon2.f90:61-110)
+    CALL READ_RESTART()
+    ST = DGBGK_SET_F(CTX, F)
+    CALL CHECKST(ST)
+  ENDIF
 
   PRINT *, 'OPENING RESIDUAL FILE = ', TRIM(IVD%ResidualFile)
   OPEN (16, FILE=IVD%ResidualFile, STATUS='UNKNOWN')
@@ -212,6 +228,10 @@ PROGRAM DGBGK_RUN
   CLOSE (16)
   CLOSE (17)
   CLOSE (18)
+  IF (RSOUT) THEN
+    CALL WRITE_RESTART()
+    CALL WRITE_GIDMESH()
+  ENDIF
   IST = DGBGK_SIZES(CTX, NSIDE, NVEL, NE, NP, NBC, IT)
   CALL DGBGK_DESTROY(CTX)
 
@@ -228,6 +248,69 @@ CONTAINS
       WRITE (*, *) 'DGBGK ERROR ', S, ' ', DGBGK_LAST_ERROR()
       STOP 1
     ENDIF
+  END SUBROUTINE
+
+  SUBROUTINE READ_RESTART()
+    INTEGER :: T1N, T2N, IV, IE, IU
+    REAL(8) :: CXR, CYR
+    IF (.NOT. ALLOCATED(F)) ALLOCATE (F(3, RORDER*TORDER, NELEM))
+    PRINT *, 'READING RESTART FILE = ', TRIM(IVD%RestartInFile)
+    OPEN (11, FILE=IVD%RestartInFile, STATUS='OLD')
+    READ (11, *) TEXT
+    READ (11, *) T1N, T2N
+    IF ((T1N .NE. NPOIN) .OR. (T2N .NE. RORDER*TORDER)) THEN
+      WRITE (*, *) 'RESTART FILE DOES NOT MATCH: NPOIN, VNPNT =', T1N, T2N
+      STOP 1
+    ENDIF
+    DO IV = 1, RORDER*TORDER
+      DO IE = 1, NELEM
+        READ (11, *) IU, CXR, CYR, F(1, IV, IE), F(2, IV, IE), F(3, IV, IE)
+      ENDDO
+    ENDDO
+    CLOSE (11)
+  END SUBROUTINE
+
+  SUBROUTINE WRITE_RESTART()
+    INTEGER :: IV, IE, IR, IZ
+    REAL(8) :: RT, TH
+    IF (.NOT. ALLOCATED(F)) ALLOCATE (F(3, RORDER*TORDER, NELEM))
+    ST = DGBGK_GET_F(CTX, F)
+    CALL CHECKST(ST)
+    PRINT *, 'OPENING NEW RESTART FILE = ', TRIM(IVD%RestartOutFile)
+    OPEN (20, FILE=IVD%RestartOutFile, STATUS='UNKNOWN')
+    WRITE (20, '(A)') 'NPOIN     VNPNT'
+    WRITE (20, *) NPOIN, '    ', RORDER*TORDER
+    DO IV = 1, RORDER*TORDER
+      ! ordinate IV = (IR-1)*TORDER + IZ (src/vspace.f90:39-49), mapped to
+      ! polar and Cartesian coordinates as src/output.f90:112-118
+      IR = (IV - 1)/TORDER + 1
+      IZ = IV - (IR - 1)*TORDER
+      RT = ETA(IR)*(IVD%rv/2) + (IVD%rv/2)
+      TH = (-1D0 + (2D0*IZ)/TORDER)*3.1416D0
+      DO IE = 1, NELEM
+        WRITE (20, 700) IE, RT*COS(TH), RT*SIN(TH), F(1, IV, IE), F(2, IV, IE), F(3, IV, IE)
+      ENDDO
+    ENDDO
+    CLOSE (20)
+700 FORMAT(I10, 2(1X, ES15.7), 3(1X, ES25.17E3))
+  END SUBROUTINE
+
+  SUBROUTINE WRITE_GIDMESH()
+    ! src/output.f90:130-153 and its formats 200-240
+    INTEGER :: Q
+    OPEN (19, FILE=IVD%GIDMeshFile, STATUS='UNKNOWN')
+    WRITE (19, '(A)') 'mesh dimension = 2 elemtype triangle nnode = 3'
+    WRITE (19, '(A)') '   COORDINATES'
+    DO Q = 1, NPOIN
+      WRITE (19, '(I10, 2(2X, F10.5))') Q, COORD(1, Q), COORD(2, Q)
+    ENDDO
+    WRITE (19, '(A)') 'end coordinates'
+    WRITE (19, '(A)') 'elements'
+    DO Q = 1, NELEM
+      WRITE (19, '(10I8)') Q, INTMA(1, Q), INTMA(2, Q), INTMA(3, Q)
+    ENDDO
+    WRITE (19, '(A)') 'end elements'
+    CLOSE (19)
   END SUBROUTINE
 
   SUBROUTINE WRITE_RESULTS(IT)

@@ -63,3 +63,32 @@ def test_fortran_host_matches_oracle(tmp_path):
     pf = o0.point_fields()
     first = np.array([list(map(float, l.split()[1:4])) for l in txt[heads[0] + 1:heads[1]]])
     assert np.allclose(first[:, 0], pf["nd"], rtol=2e-5)
+
+
+@pytest.mark.gpu
+def test_fortran_host_restart_chain(tmp_path):
+    """Six steps straight, and three steps written to a restart file then
+    three more from it (IVD%RS = 1): the restart files written at the end
+    hold the same f, bit for bit (a step depends on f alone).  The GiD
+    mesh file holds the mesh."""
+    from dgbgk import io
+    assert os.path.exists(HOST), "Fortran host not built"
+    case = cylinder_small(elems=800)
+    E, V, P = case.mesh.nelem, case.nvel, case.mesh.npoin
+    print(f"fortran host restart chain E={E} V={V}", flush=True)
+
+    def run(d, nt, restart_in=None):
+        os.makedirs(d, exist_ok=True)
+        deck = io.write_run_dir(case, d, ntime=nt, restart_in=restart_in)
+        out = subprocess.run([HOST, deck, "--restart-out"], cwd=d, capture_output=True,
+                             text=True, timeout=120)
+        assert out.returncode == 0, out.stdout + out.stderr
+        return io.read_restart(os.path.join(d, "RESTART_OUT.RES"), E, V, P)
+
+    a = run(str(tmp_path / "a"), 6)
+    run(str(tmp_path / "b"), 3)
+    c = run(str(tmp_path / "c"), 3, restart_in=str(tmp_path / "b" / "RESTART_OUT.RES"))
+    assert np.array_equal(a, c)
+    coord, intma = io.read_gid_mesh(str(tmp_path / "a" / "GIDMESH.RES"))
+    assert np.array_equal(intma, case.mesh.intma)
+    assert np.allclose(coord, case.mesh.coord, atol=6e-6)

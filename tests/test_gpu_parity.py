@@ -202,3 +202,37 @@ def test_change_residual_is_max_step_difference(precision):
         # the reference residual counts the increment before the reset
         assert r["sumres"] >= r["change"] * (1 - 1e-6)
         f0 = f1
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_moment_pass_and_prefetch_depth(precision):
+    """The update kernel's fused first moment pass (FP64 partial sums per
+    ordinate chunk) against the two-pass moments kernel, which sums in the
+    same order, prefetch depth 2 against 1, and 512-slot tiles against
+    256: the same arithmetic, so f and the node fields are bit-identical."""
+    import os
+    from dgbgk.solver import Solver
+    case = cylinder_small(elems=800)
+    print(f"case fused moments precision={precision}", flush=True)
+    runs = {}
+    for name, env in (("fused", {}), ("two_pass", {"DGBGK_FUSE_MOMENTS": "0"}),
+                      ("depth2", {"DGBGK_PREFETCH": "2"}), ("slots512", {"DGBGK_SLOTS": "512"})):
+        old = {k: os.environ.get(k) for k in env}
+        os.environ.update(env)
+        try:
+            s = Solver(case, precision=precision, vchunk=24)
+        finally:
+            for k, v in old.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+        s.step(3)
+        runs[name] = (s.get_f(), s.node_fields())
+        s.close()
+    f0, n0 = runs["fused"]
+    for name in ("two_pass", "depth2", "slots512"):
+        f1, n1 = runs[name]
+        assert np.array_equal(f1, f0), name
+        for k in ("disnd", "disux", "disuy", "disps", "mu"):
+            assert np.array_equal(n1[k], n0[k]), (name, k)
