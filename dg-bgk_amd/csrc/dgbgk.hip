@@ -697,7 +697,7 @@ __global__ void __launch_bounds__(256) k_points(const double* __restrict__ mom,
 
 // Step epilogue: GETRES max, mass, GETFOR (src/getfor.f90:29-49) on the
 // gathered point pressure, the residual record, and ALOTIM2's next DTE.
-__global__ void __launch_bounds__(256) k_final(const double* __restrict__ res_part, int nres,
+__global__ void __launch_bounds__(1024) k_final(const double* __restrict__ res_part, int nres,
                                                const double* __restrict__ mu_part,
                                                const double* __restrict__ mass_part, int nmom,
                                                const int* __restrict__ fside, int nf,
@@ -706,7 +706,7 @@ __global__ void __launch_bounds__(256) k_final(const double* __restrict__ res_pa
                                                double* __restrict__ scal,
                                                StepRec* __restrict__ rec, int rec_cap,
                                                int record) {
-  __shared__ double s_red[8];
+  __shared__ double s_red[16];
   double r = 0.0, rc = 0.0, m = 0.0, ms = 0.0, lift = 0.0, drag = 0.0;
   for (int i = threadIdx.x; i < nres; i += blockDim.x) {
     r = fmax(r, res_part[i]);
@@ -906,7 +906,7 @@ int launch_moments(Ctx& c, bool partials = false) {
 }
 
 int launch_final(Ctx& c, int record) {
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, c.stream, c.res_part,
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(1024), 0, c.stream, c.res_part,
                      c.nblk_upd + c.nblk_wall, c.mu_part, c.mass_part, c.nblk_mom, c.fside,
                      c.nforce, c.coord, c.ps_glob, c.scal, c.rec, c.rec_cap, record);
   HIPCK(hipGetLastError());
@@ -1273,7 +1273,7 @@ int dgbgk_setup_create(int nelem, int npoin, int nboun, const int* intma, const 
   if (!s) return fail(kErrNoMem, "out of host memory");
   std::string msg;
   int st = prepare(nelem, npoin, nboun, intma, coord, bsido, rorder, torder, eta, w,
-                   to_params(prm), kSlots, s->p, msg);
+                   to_params(prm), kSlots, kSlots, s->p, msg);
   if (st) {
     delete s;
     return fail(st, msg);
@@ -1327,12 +1327,16 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   if (!h) return fail(kErrNoMem, "out of host memory");
   Ctx& c = h->c;
   std::string msg;
-  // update tile size: 256 slots; 512 (half the halo per owned element)
-  // on request
+  // update tiles: 256 slots, owned elements up to the slots left by the
+  // halo.  512 slots, or an owned cap (a multiple of 64 keeps the halo in
+  // its own wavefront, which then skips phase B), on request; both
+  // measured slower on gfx950.
   c.slots = kSlots;
   if (const char* ps = std::getenv("DGBGK_SLOTS")) c.slots = std::atoi(ps) == 512 ? 512 : 256;
+  int owned_cap = c.slots;
+  if (const char* po = std::getenv("DGBGK_TILE_OWNED")) owned_cap = std::atoi(po);
   int st = prepare(nelem, npoin, nboun, intma, coord, bsido, rorder, torder, eta, w,
-                   to_params(prm), c.slots, c.p, msg);
+                   to_params(prm), c.slots, owned_cap, c.p, msg);
   if (st) {
     delete h;
     return fail(st, msg);
@@ -1341,10 +1345,11 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   c.gs = prm->order == DGBGK_ORDER_REFERENCE ? 1 : 0;
   if (const char* ng = std::getenv("DGBGK_NO_GRAPH")) c.use_graph = ng[0] == '0';
   c.tsz = c.prec ? 4 : 8;
-  // ordinates of f in flight per update thread beyond the one computed
-  // (deeper queues cost more in occupancy than they gain on gfx950), and
-  // whether the update kernel also sums the first moment pass
-  c.depth = 1;
+  // ordinates of f in flight per update thread beyond the one computed:
+  // two in FP32, one in FP64 (where a second slot costs a wavefront of
+  // occupancy), and whether the update kernel also sums the first moment
+  // pass
+  c.depth = c.prec ? 2 : 1;
   if (const char* pd = std::getenv("DGBGK_PREFETCH")) {
     const int d = std::atoi(pd);
     if (d >= 1 && d <= 2) c.depth = d;

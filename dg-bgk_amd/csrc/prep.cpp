@@ -158,7 +158,7 @@ uint64_t pair_key(int a, int b) {
 
 int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coord,
             const int* bsido, int rorder, int torder, const double* eta,
-            const double* w, const Params& prm, int tile_slots, Prep& p,
+            const double* w, const Params& prm, int tile_slots, int tile_owned, Prep& p,
             std::string& msg) {
   if (nelem < 1 || npoin < 3 || nboun < 0 || rorder < 1 || torder < 1) {
     msg = "invalid sizes";
@@ -461,8 +461,11 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
     std::vector<int> pos(p.pt_ptr.begin(), p.pt_ptr.end() - 1);
     for (int k = 0; k < 3 * E; ++k) p.pt_node[pos[p.intma[k] - 1]++] = k;
   }
-  // tiles: contiguous owned ranges whose owned + halo elements fit the slots
+  // tiles: contiguous owned ranges of at most tile_owned elements whose
+  // owned + halo elements fit the slots (tile_owned a multiple of the
+  // wavefront size keeps halo slots out of the owned wavefronts)
   {
+    if (tile_owned <= 0 || tile_owned > tile_slots) tile_owned = tile_slots;
     p.nbr_slot.assign(3 * (size_t)E, -1);
     std::vector<int> mark(E, -1), hidx(E, -1);
     int e0 = 0;
@@ -481,7 +484,7 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
         }
         int rem = (mark[e1] == tile) ? 1 : 0;
         int nown = e1 - e0 + 1;
-        if (nown + nh + add - rem > tile_slots) break;
+        if (nown > tile_owned || nown + nh + add - rem > tile_slots) break;
         if (rem) { mark[e1] = -2; --nh; }
         for (int k = 0; k < 3; ++k) {
           int nb = p.edges[3 * (size_t)e1 + k].nbr;

@@ -88,7 +88,7 @@ struct Prep {
 // Builds everything; returns a Status and fills msg on error.
 int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coord,
             const int* bsido, int rorder, int torder, const double* eta,
-            const double* w, const Params& prm, int tile_slots, Prep& out,
+            const double* w, const Params& prm, int tile_slots, int tile_owned, Prep& out,
             std::string& msg);
 
 // The reference's preprocessing steps, exposed for bit-exact checks.

@@ -41,11 +41,14 @@ def test_setup_tables_match_oracle_bit_for_bit(make, lookup):
 
 def test_tiles_cover_every_element_once():
     """The update kernel's tiles: contiguous owned ranges covering the mesh,
-    owned + halo within the 256 LDS slots."""
+    owned + halo within the 256 LDS slots (setup fails with DGBGK_ERR_TILE
+    otherwise)."""
     from dgbgk.solver import Setup
     for case in (c1(), cylinder_small(elems=6000)):
         s = Setup(case)
         assert s.ntiles >= (case.mesh.nelem + 255) // 256
+        # halo slots take at most a third of a tile on these meshes
+        assert s.ntiles <= (case.mesh.nelem + 169) // 170
 
 
 def _err(case, **kw):
