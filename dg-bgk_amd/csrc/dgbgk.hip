@@ -558,15 +558,12 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
     }
     double sp = 0, cp = 0, st = 0, ct = 0;
     const bool dens = n >= 1e-20;
-    int v = 0;
-    for (; v + 4 <= V; v += 4) {
-      double fv[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) fv[j] = (double)fp[(size_t)(v + j) * N3];
+    // second pass, four ordinates at a time
+    auto block4 = [&](const T (&raw)[4], int v) {
       double bp = 0, bt = 0;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const double fw = fv[j] * ldc(wv, v + j);
+        const double fw = (double)raw[j] * ldc(wv, v + j);
         const double dx = ldc(cxb, v + j) - ux, dy = ldc(cyb, v + j) - uy;
         const double ex = ux - ldc(cxa, v + j), ey = uy - ldc(cya, v + j);
         const double e2 = ex * ex + ey * ey;
@@ -585,6 +582,29 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
         nsum<true>(sp, cp, bp);
         if (dens) nsum<true>(st, ct, bt);
       }
+    };
+    // ping-pong over two register sets: the next four ordinates' loads are
+    // in flight while the current four are summed (loads clamped to the
+    // last ordinate, so none is conditional)
+    int v = 0;
+    if (V >= 8) {
+      T ra[4], rb[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) ra[j] = fp[(size_t)j * N3];
+      for (; v + 8 <= V; v += 8) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) rb[j] = fp[(size_t)(v + 4 + j) * N3];
+        block4(ra, v);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ra[j] = fp[(size_t)min(v + 8 + j, V - 1) * N3];
+        block4(rb, v + 4);
+      }
+    }
+    for (; v + 4 <= V; v += 4) {
+      T rc[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) rc[j] = fp[(size_t)(v + j) * N3];
+      block4(rc, v);
     }
     for (; v < V; ++v) {
       const double fw = (double)fp[(size_t)v * N3] * ldc(wv, v);
