@@ -43,7 +43,10 @@ import numpy as np
 from . import _lib, cases, meshgen
 
 FEATURES = ("x", "y", "wall_distance", "mach", "kn")
-TARGETS = ("rho", "u/U0", "v/U0", "T")
+# network outputs: the departures from the freestream (rho = T = 1 in these
+# units), scaled by Mach^2 for rho and T (they vanish like Mach^2), so that
+# low-Mach states are learned to the same relative accuracy as high-Mach ones
+TARGETS = ("(rho-1)/M^2", "u/U0", "v/U0", "(T-1)/M^2")
 FAR_FIELD = (1, 1, 1, 1)
 KN_MIN = 0.05
 
@@ -79,18 +82,35 @@ def features(mesh, wd, s: Sample) -> np.ndarray:
                      np.full(n, math.log10(s.kn))], axis=1).astype(np.float32)
 
 
-def targets(fields: dict, s: Sample) -> np.ndarray:
-    """Network outputs, scaled to O(1): rho, u/U0, v/U0, T."""
+def fields_scaled(fields: dict, s: Sample) -> np.ndarray:
+    """What the training data stores per point: rho, u/U0, v/U0, T."""
     u0 = cases.mach_to_u(s.mach)
     return np.stack([fields["rho"], fields["uvel"] / u0, fields["vvel"] / u0,
                      fields["temp"]], axis=1).astype(np.float32)
 
 
+def net_targets(y: np.ndarray, s: Sample) -> np.ndarray:
+    """Network outputs (TARGETS) from stored rho, u/U0, v/U0, T."""
+    m2 = s.mach * s.mach
+    y = np.asarray(y, np.float64)
+    return np.stack([(y[:, 0] - 1.0) / m2, y[:, 1], y[:, 2], (y[:, 3] - 1.0) / m2],
+                    axis=1).astype(np.float32)
+
+
+def targets(fields: dict, s: Sample) -> np.ndarray:
+    """Network outputs (TARGETS) from solver point fields."""
+    return net_targets(fields_scaled(fields, s), s)
+
+
 def untarget(y: np.ndarray, s: Sample):
+    """Point-wise rho, u, v, T from network outputs (rho and T clamped
+    positive)."""
     u0 = cases.mach_to_u(s.mach)
-    rho = np.maximum(y[:, 0].astype(np.float64), 1e-3)
-    temp = np.maximum(y[:, 3].astype(np.float64), 1e-3)
-    return rho, (y[:, 1] * u0).astype(np.float64), (y[:, 2] * u0).astype(np.float64), temp
+    m2 = s.mach * s.mach
+    y = np.asarray(y, np.float64)
+    rho = np.maximum(1.0 + m2 * y[:, 0], 1e-3)
+    temp = np.maximum(1.0 + m2 * y[:, 3], 1e-3)
+    return rho, y[:, 1] * u0, y[:, 2] * u0, temp
 
 
 def run_to_steady(case, tol: float, max_steps: int, precision=_lib.DGBGK_FP64, seed=None,
@@ -128,6 +148,10 @@ def run_to_steady(case, tol: float, max_steps: int, precision=_lib.DGBGK_FP64, s
     fields = s.point_fields()
     lift, drag = s.forces()
     s.close()
+    # a state with vacuum points (f clamped to zero over a region) has not
+    # reached the flow's steady state, however still it has become
+    if conv is not None and not np.all(fields["nd"] > 0.0):
+        conv, bad = None, True
     return dict(steps=conv if conv is not None else steps, converged=conv is not None,
                 diverged=bad,
                 wall_s=wall * (conv / steps if conv else 1.0), steps_run=steps,
@@ -172,7 +196,7 @@ def generate(elems, n, seed, rorder, torder, tol, max_steps, data, time_budget=N
         d["wall_s"] = np.append(d["wall_s"], r["wall_s"])
         d["drag"] = np.append(d["drag"], r["drag"])
         d["converged"] = np.append(d["converged"], r["converged"])
-        d["y"] = np.concatenate([d["y"], targets(r["fields"], s)[None]])
+        d["y"] = np.concatenate([d["y"], fields_scaled(r["fields"], s)[None]])
         _save(data, d)
         log(f"  case {i}: M={s.mach:.3f} Kn={s.kn:.4f} {r['steps']} steps "
             f"{'converged' if r['converged'] else 'NOT converged'} {r['wall_s']:.2f}s "
@@ -188,9 +212,9 @@ def train(elems, data, params, steps, width=256, hidden=6, batch=65536, lr=1e-3,
     wd = meshgen.wall_distance(mesh)
     d = _load(data)
     keep = d["converged"]
-    X = np.concatenate([features(mesh, wd, Sample(m, k))
-                        for m, k in zip(d["mach"][keep], d["kn"][keep])])
-    Y = d["y"][keep].reshape(-1, 4)
+    samples = [Sample(m, k) for m, k in zip(d["mach"][keep], d["kn"][keep])]
+    X = np.concatenate([features(mesh, wd, s) for s in samples])
+    Y = np.concatenate([net_targets(y, s) for y, s in zip(d["y"][keep], samples)])
     log(f"  training set: {int(keep.sum())} steady states, {len(X)} points")
     m = MLP(fin=X.shape[1], width=width, hidden=hidden, fout=Y.shape[1], batch=batch)
     m.init(seed=seed)
@@ -221,10 +245,15 @@ def load_model(params):
     return m
 
 
-def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print):
-    """Per held-out case: FP64 and FP32-bulk runs, each from the freestream
-    and from the network's state, all judged by the FP64 freestream run's
-    step-1 change."""
+RUNS = {"warm64": (_lib.DGBGK_FP64, True), "cold32": (_lib.DGBGK_FP32, False),
+        "warm32": (_lib.DGBGK_FP32, True)}
+
+
+def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print,
+             runs=("warm64", "cold32", "warm32")):
+    """Per held-out case: the FP64 run from the freestream (cold64) and the
+    requested others (FP64/FP32-bulk, from the freestream or from the
+    network's state), all judged by cold64's step-1 change."""
     m = load_model(params)
     mesh = mesh_for(elems)
     wd = meshgen.wall_distance(mesh)
@@ -234,13 +263,13 @@ def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print):
         pred = untarget(m.forward(features(mesh, wd, s)), s)
         cold64 = run_to_steady(case, tol, max_steps, _lib.DGBGK_FP64)
         norm = cold64["first_change"]
-        runs = dict(cold64=cold64,
-                    warm64=run_to_steady(case, tol, max_steps, _lib.DGBGK_FP64, pred, norm),
-                    cold32=run_to_steady(case, tol, max_steps, _lib.DGBGK_FP32, None, norm),
-                    warm32=run_to_steady(case, tol, max_steps, _lib.DGBGK_FP32, pred, norm))
+        done = dict(cold64=cold64)
+        for k in runs:
+            prec, warm = RUNS[k]
+            done[k] = run_to_steady(case, tol, max_steps, prec, pred if warm else None, norm)
         ref = cold64["fields"]
         rec = dict(mach=s.mach, kn=s.kn, norm=norm)
-        for k, r in runs.items():
+        for k, r in done.items():
             rec[k] = dict(steps=int(r["steps"]), converged=bool(r["converged"]),
                           wall_s=float(r["wall_s"]), drag=float(r["drag"]),
                           drag_rel=float(abs(r["drag"] - cold64["drag"]) /
@@ -254,26 +283,26 @@ def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print):
         log(f"  held-out M={s.mach:.3f} Kn={s.kn:.4f}: " + ", ".join(
             f"{k} {rec[k]['steps']} steps {rec[k]['wall_s']:.2f}s"
             f"{'' if rec[k]['converged'] else ' (not converged)'}"
-            for k in runs) + f"; warm32 drag rel {rec['warm32']['drag_rel']:.1e}; "
-            f"prediction rho rel L2 {rec['pred_rho_rel_l2']:.1e}")
+            for k in done) + "; drag rel " + ", ".join(
+            f"{k} {rec[k]['drag_rel']:.1e}" for k in runs) +
+            f"; prediction rho rel L2 {rec['pred_rho_rel_l2']:.1e}")
     return out
 
 
 def summarise(res):
+    kinds = [k for k in ("cold64", "warm64", "cold32", "warm32") if k in res[0]]
+
     def med(key, a, b):
         return float(np.median([r[a][key] / max(r[b][key], 1e-30) for r in res]))
-    ok = [r for r in res if all(r[k]["converged"] for k in ("cold64", "warm64", "cold32",
-                                                                "warm32"))]
-    return dict(
-        n=len(res), all_converged=len(ok),
-        speedup_steps_warm64_vs_cold64=med("steps", "cold64", "warm64"),
-        speedup_steps_warm32_vs_cold64=med("steps", "cold64", "warm32"),
-        speedup_wall_warm32_vs_cold64=med("wall_s", "cold64", "warm32"),
-        speedup_wall_warm64_vs_cold64=med("wall_s", "cold64", "warm64"),
-        speedup_wall_cold32_vs_cold64=med("wall_s", "cold64", "cold32"),
-        max_drag_rel_warm32=float(max(r["warm32"]["drag_rel"] for r in res)),
-        max_drag_rel_warm64=float(max(r["warm64"]["drag_rel"] for r in res)),
-        median_pred_rho_rel_l2=float(np.median([r["pred_rho_rel_l2"] for r in res])))
+    ok = [r for r in res if all(r[k]["converged"] for k in kinds)]
+    out = dict(n=len(res), all_converged=len(ok))
+    for k in kinds[1:]:
+        out[f"speedup_steps_{k}_vs_cold64"] = med("steps", "cold64", k)
+        out[f"speedup_wall_{k}_vs_cold64"] = med("wall_s", "cold64", k)
+        out[f"median_drag_rel_{k}"] = float(np.median([r[k]["drag_rel"] for r in res]))
+        out[f"max_drag_rel_{k}"] = float(max(r[k]["drag_rel"] for r in res))
+    out["median_pred_rho_rel_l2"] = float(np.median([r["pred_rho_rel_l2"] for r in res]))
+    return out
 
 
 def main(argv=None):
@@ -294,6 +323,8 @@ def main(argv=None):
     ap.add_argument("--data", default="gpurun_out/warmstart_train.npz")
     ap.add_argument("--params", default="gpurun_out/warmstart_params.npz")
     ap.add_argument("--out", default="gpurun_out/warmstart_eval.json")
+    ap.add_argument("--runs", default="warm64,cold32,warm32",
+                    help="runs besides the FP64 freestream run, of warm64, cold32, warm32")
     a = ap.parse_args(argv)
     log = lambda *x: print(*x, flush=True)  # noqa: E731
     os.makedirs(os.path.dirname(os.path.abspath(a.data)), exist_ok=True)
@@ -310,7 +341,7 @@ def main(argv=None):
                             loss_last=float(losses[-50:].mean()))))
     else:
         res = evaluate(a.elems, a.params, a.n, a.seed, a.rorder, a.torder, a.tol,
-                       a.max_steps, log)
+                       a.max_steps, log, runs=tuple(x for x in a.runs.split(",") if x))
         summ = summarise(res)
         with open(a.out, "w") as fh:
             json.dump(dict(elems=a.elems, ordinates=a.rorder * a.torder, tol=a.tol,

@@ -11,8 +11,8 @@ pytestmark = pytest.mark.gpu
 
 
 def test_trained_network_shortens_the_run_to_steady_state(tmp_path):
-    from dgbgk import _lib, meshgen, warmstart
-    elems, rorder, torder, tol = 1200, 8, 8, 1e-5
+    from dgbgk import meshgen, warmstart
+    elems, rorder, torder, tol = 1200, 8, 16, 1e-5
     train_set = [warmstart.Sample(m, k) for m in (0.5, 0.75, 1.0) for k in (0.1, 0.3)]
     held_out = warmstart.Sample(0.65, 0.18)
     mesh = warmstart.mesh_for(elems)
