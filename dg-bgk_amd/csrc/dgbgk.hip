@@ -244,8 +244,17 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
 // kMom: also accumulate the first moment pass of f^{k+1} (n, n u, n v per
 // node over the block's ordinates) into msum.
 // kS: slots (threads) per block, the tile size the host built for.
+// Minimum wavefronts per SIMD the FP32 update kernel is compiled for (1:
+// no constraint, the compiler's own register allocation).
+#ifndef DGBGK_FP32_MIN_WAVES
+#define DGBGK_FP32_MIN_WAVES 1
+#endif
+template <typename T>
+constexpr int kMinWaves = sizeof(T) == 4 ? DGBGK_FP32_MIN_WAVES : 1;
+
 template <typename T, int kDepth, bool kMom, int kS>
-__global__ void __launch_bounds__(kS) k_update(UpdArgs<T> a) {
+__global__ void __launch_bounds__(kS) __attribute__((amdgpu_waves_per_eu(kMinWaves<T>, 8)))
+k_update(UpdArgs<T> a) {
   constexpr int kQ = kDepth + 1;
   __shared__ __align__(16) T s_f[2][kS * 4];
   __shared__ double s_red[8];

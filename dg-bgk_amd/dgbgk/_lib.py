@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libdgbgk.so")
+# DGBGK_LIB: another build of the same library (for comparing builds)
+LIB_PATH = os.environ.get("DGBGK_LIB") or os.path.join(PKG, "libdgbgk.so")
 
 DGBGK_FP64, DGBGK_FP32 = 0, 1
 LOOKUP_REFERENCE, LOOKUP_CORRECTED = 0, 1

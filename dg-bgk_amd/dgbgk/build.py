@@ -36,7 +36,8 @@ def _run(cmd, cwd=None):
     return r
 
 
-def build_lib(verbose: bool = False) -> str:
+def build_lib(verbose: bool = False, out: str = LIB, defines=()) -> str:
+    """Builds libdgbgk.so (or another build at `out` with extra -D defines)."""
     os.makedirs(BUILD, exist_ok=True)
     prep_o = os.path.join(BUILD, "prep.o")
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
@@ -46,15 +47,16 @@ def build_lib(verbose: bool = False) -> str:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
-        o = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC] + HIP_FLAGS + ["-c", path, "-o", o])
+        tag = "" if not defines else "_" + "_".join(d.replace("=", "") for d in defines)
+        o = os.path.join(BUILD, src.replace(".hip", tag + ".o"))
+        _run([HIPCC] + HIP_FLAGS + [f"-D{d}" for d in defines] + ["-c", path, "-o", o])
         objs.append(o)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", out)
+    return out
 
 
 def build_fortran_host(verbose: bool = False) -> str | None:
