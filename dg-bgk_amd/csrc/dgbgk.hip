@@ -218,19 +218,27 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
   // GETINC + ADTHEM + INFLOW (src/getinc.f90:30-42, src/adthem.f90:109-118,
   // src/inflow.f90:165-181); mdt = 6/RJ * DTE
   T* fo = a.fnxt + (size_t)v * N3 + 3 * (size_t)s.e;
+  T d[3], fn[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const T d = s.mdt * rhs[k] + cold[k];
-    res = fmax(res, fabs(d));
-    T fn = f[k] + d;
-    fn = fn < T(0) ? T(0) : fn;
-    if ((s.mask >> k) & 1) fn = fin;
-    chg = fmax(chg, fabs(fn - f[k]));
-    fo[k] = fn;
+    d[k] = s.mdt * rhs[k] + cold[k];
+    fn[k] = f[k] + d[k];
+    fn[k] = fn[k] < T(0) ? T(0) : fn[k];  // NaN stays NaN, as in the reference
+  }
+  if (s.mask) {  // inflow nodes: rare, so the selects sit behind a branch
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if ((s.mask >> k) & 1) fn[k] = fin;
+  }
+  res = fmax(res, fmax(fmax(fabs(d[0]), fabs(d[1])), fabs(d[2])));
+  chg = fmax(chg, fmax(fmax(fabs(fn[0] - f[0]), fabs(fn[1] - f[1])), fabs(fn[2] - f[2])));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    fo[k] = fn[k];
     if (kMom) {
       // MACROS' first pass (src/macros.f90:93-108) on f^{k+1}, this chunk's
       // ordinates, in FP64
-      const double fw = (double)fn * wv;
+      const double fw = (double)fn[k] * wv;
       m0[k] += fw;
       m1[k] += cx * fw;
       m2[k] += cy * fw;

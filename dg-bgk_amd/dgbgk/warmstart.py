@@ -114,7 +114,8 @@ def untarget(y: np.ndarray, s: Sample):
 
 
 def run_to_steady(case, tol: float, max_steps: int, precision=_lib.DGBGK_FP64, seed=None,
-                  norm: float | None = None, chunk: int = 1000):
+                  norm: float | None = None, chunk: int = 1000, log=None,
+                  log_every: int = 20000):
     """Steps until change / norm < tol (norm: this run's step-1 change unless
     given).  Wall time covers the steps up to convergence, scaled from the
     timed chunks.  Returns steps, wall seconds, the change history, point
@@ -145,6 +146,9 @@ def run_to_steady(case, tol: float, max_steps: int, precision=_lib.DGBGK_FP64, s
         if bad:
             conv = None
             break
+        if log is not None and steps % log_every == 0:
+            log(f"    ... {steps} steps, change {hist[-1]:.3e} "
+                f"(target {tol * (norm if norm is not None else first):.3e})")
     fields = s.point_fields()
     lift, drag = s.forces()
     s.close()
@@ -189,7 +193,7 @@ def generate(elems, n, seed, rorder, torder, tol, max_steps, data, time_budget=N
         if time_budget and time.perf_counter() - t0 > time_budget:
             log(f"  time budget reached after {i} cases")
             break
-        r = run_to_steady(cylinder_case(mesh, s, rorder, torder), tol, max_steps)
+        r = run_to_steady(cylinder_case(mesh, s, rorder, torder), tol, max_steps, log=log)
         d["mach"] = np.append(d["mach"], s.mach)
         d["kn"] = np.append(d["kn"], s.kn)
         d["steps"] = np.append(d["steps"], r["steps"])
@@ -261,12 +265,13 @@ def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print,
     for s in draw(n, seed):
         case = cylinder_case(mesh, s, rorder, torder)
         pred = untarget(m.forward(features(mesh, wd, s)), s)
-        cold64 = run_to_steady(case, tol, max_steps, _lib.DGBGK_FP64)
+        cold64 = run_to_steady(case, tol, max_steps, _lib.DGBGK_FP64, log=log)
         norm = cold64["first_change"]
         done = dict(cold64=cold64)
         for k in runs:
             prec, warm = RUNS[k]
-            done[k] = run_to_steady(case, tol, max_steps, prec, pred if warm else None, norm)
+            done[k] = run_to_steady(case, tol, max_steps, prec, pred if warm else None, norm,
+                                    log=log)
         ref = cold64["fields"]
         rec = dict(mach=s.mach, kn=s.kn, norm=norm)
         for k, r in done.items():

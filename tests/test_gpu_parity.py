@@ -236,3 +236,25 @@ def test_fused_moment_pass_and_prefetch_depth(precision):
         assert np.array_equal(f1, f0), name
         for k in ("disnd", "disux", "disuy", "disps", "mu"):
             assert np.array_equal(n1[k], n0[k]), (name, k)
+
+
+@pytest.mark.timeout(600)
+def test_fp32_bulk_unit_square_2000_steps():
+    """The precision form over the unit-square run's full length: FP32 f,
+    flux and relaxation with FP64 moments, 2000 steps, against the FP64 CPU
+    oracle: relative L2 of the point fields rho, u, v, T within 1e-5 and
+    the total mass of every step within 1e-6."""
+    from dgbgk import _lib
+    s, o = _pair(c1(), precision=_lib.DGBGK_FP32)
+    n = 2000
+    print(f"case c1 fp32 {n} steps (the oracle takes about a minute)", flush=True)
+    rs, ro = s.step(n), o.step(n)
+    pf, op = s.point_fields(), o.point_fields()
+    errs = {k: rel_l2(pf[k], op[k]) for k in ("rho", "temp", "ps")}
+    errs["u,v"] = rel_l2_vec(pf["uvel"], pf["vvel"], op["uvel"], op["vvel"])
+    drift = max(abs(a["mass"] - b["mass"]) / abs(b["mass"]) for a, b in zip(rs, ro))
+    print("relative L2 after 2000 steps:", {k: f"{v:.2e}" for k, v in errs.items()},
+          f"max per-step mass difference {drift:.2e}", flush=True)
+    for k, v in errs.items():
+        assert v < FP32_TOL, (k, v)
+    assert drift < 1e-6
