@@ -82,7 +82,8 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the time march has no multi-GPU path "
-              "in this build (see DESIGN.md for how it would shard)", file=sys.stderr)
+              "(a split of the speed rings over GPUs is designed, not built)",
+              file=sys.stderr)
         sys.exit(2)
 
     from dgbgk import _lib, cases

@@ -238,23 +238,104 @@ def test_fused_moment_pass_and_prefetch_depth(precision):
             assert np.array_equal(n1[k], n0[k]), (name, k)
 
 
-@pytest.mark.timeout(600)
-def test_fp32_bulk_unit_square_2000_steps():
-    """The precision form over the unit-square run's full length: FP32 f,
-    flux and relaxation with FP64 moments, 2000 steps, against the FP64 CPU
-    oracle: relative L2 of the point fields rho, u, v, T within 1e-5 and
-    the total mass of every step within 1e-6."""
-    from dgbgk import _lib
-    s, o = _pair(c1(), precision=_lib.DGBGK_FP32)
-    n = 2000
-    print(f"case c1 fp32 {n} steps (the oracle takes about a minute)", flush=True)
-    rs, ro = s.step(n), o.step(n)
+def _c1_gauss_legendre(nsteps=2000):
+    """The unit-square deck with 8 Gauss-Legendre speeds in place of its
+    midpoint speeds (same mesh, flow and ordinate count).  With the deck's own
+    midpoint speeds the reference method does not stay bounded past a few
+    hundred steps: the FP64 oracle's mass reads 1.04 at step 100, 2.98 at
+    step 1700 and 1.8e34 at step 2000.  With these speeds it stays bounded
+    (mass 0.967 at step 2000, rho in [0.89, 1.0])."""
+    from dgbgk import cases
+    base = c1()
+    eta, w = cases.gauss_legendre_speeds(len(base.eta))
+    return cases.Case("c1_gauss_legendre", base.mesh, eta, w, base.torder, base.prm,
+                      nsteps=nsteps)
+
+
+def _point_errors(s, o):
     pf, op = s.point_fields(), o.point_fields()
     errs = {k: rel_l2(pf[k], op[k]) for k in ("rho", "temp", "ps")}
     errs["u,v"] = rel_l2_vec(pf["uvel"], pf["vvel"], op["uvel"], op["vvel"])
+    return errs
+
+
+def test_fp32_bulk_one_step_error():
+    """The FP32-bulk step's own error, free of how a deck amplifies it: from
+    the oracle's state after 100 steps (stored in FP32 on the device), one
+    device step against one FP64 oracle step.  Rounding f to FP32 alone is
+    0.29 ulp RMS (1.7e-8 relative); the step stays within 1e-7 RMS and moves
+    the total mass within 1e-8 of the oracle's step."""
+    from dgbgk import _lib
+    case = _c1_gauss_legendre()
+    s, o = _pair(case, precision=_lib.DGBGK_FP32)
+    print("case c1 Gauss-Legendre fp32, one step from the oracle's step-100 state", flush=True)
+    o.step(100)
+    f0 = o.get_f()
+    s.set_f(f0)
+    rs, ro = s.step(1), o.step(1)
+    fs, fo = s.get_f(), o.get_f()
+    rms = np.sqrt(((fs - fo) ** 2).mean() / (fo ** 2).mean())
+    dm = abs(rs[0]["mass"] - ro[0]["mass"]) / ro[0]["mass"]
+    print(f"one-step rms {rms:.2e}, mass difference {dm:.2e}", flush=True)
+    assert rms < 1e-7
+    assert dm < 1e-8
+
+
+def test_fp32_bulk_c1_full_run():
+    """The unit-square deck over its own length (200 steps) in the FP32-bulk
+    form against the FP64 oracle: rho, T, p and the total mass of every step
+    within 1e-5.  The relative L2 of (u, v) is 14 times that of rho in both
+    precisions on this deck (u is 0.1 of the thermal speed; FP64 against the
+    oracle: 2.2e-15 and 3.0e-14), so it is held to 5e-5."""
+    from dgbgk import _lib
+    s, o = _pair(c1(), precision=_lib.DGBGK_FP32)
+    print("case c1 fp32, 200 steps", flush=True)
+    rs, ro = s.step(200), o.step(200)
+    errs = _point_errors(s, o)
     drift = max(abs(a["mass"] - b["mass"]) / abs(b["mass"]) for a, b in zip(rs, ro))
-    print("relative L2 after 2000 steps:", {k: f"{v:.2e}" for k, v in errs.items()},
+    print("relative L2 after 200 steps:", {k: f"{v:.2e}" for k, v in errs.items()},
           f"max per-step mass difference {drift:.2e}", flush=True)
-    for k, v in errs.items():
-        assert v < FP32_TOL, (k, v)
-    assert drift < 1e-6
+    for k in ("rho", "temp", "ps"):
+        assert errs[k] < FP32_TOL, (k, errs[k])
+    assert errs["u,v"] < 5 * FP32_TOL
+    assert drift < FP32_TOL
+
+
+@pytest.mark.timeout(600)
+def test_unit_square_2000_steps():
+    """2000 steps of the unit square (Gauss-Legendre speeds) on the device
+    in both precisions against one FP64 oracle run.  FP64: point fields
+    within 1e-10 and the mass of every step within 1e-12.  This deck
+    amplifies a perturbation about 1e4 times over 2000 steps (FP64 against
+    the oracle: 1e-16 at step 1, 2.5e-13 at step 2000), so the FP32-bulk
+    form is reported and held to stay bounded: storing f in FP32, with every
+    operation in FP64, already gives rho 0.8-1.5e-5 and (u, v) 0.7-1.2e-4
+    after 2000 steps here; the FP32-bulk form gives rho 7.4e-5, T 3.7e-5,
+    (u, v) 5e-4, mass 7.4e-5."""
+    from dgbgk import _lib
+    from dgbgk.solver import Solver
+    from oracle_py import OracleRun
+    case = _c1_gauss_legendre()
+    n = 2000
+    o = OracleRun(case, lookup=1)
+    print(f"case c1 Gauss-Legendre {n} steps: oracle (about a minute)", flush=True)
+    ro = o.step(n)
+    assert all(np.isfinite(r["mass"]) for r in ro)
+    assert 0.9 < ro[-1]["mass"] < 1.0
+    for prec, name in ((_lib.DGBGK_FP64, "fp64"), (_lib.DGBGK_FP32, "fp32")):
+        s = Solver(case, precision=prec, lookup=1)
+        rs = s.step(n)
+        errs = _point_errors(s, o)
+        drift = max(abs(a["mass"] - b["mass"]) / abs(b["mass"]) for a, b in zip(rs, ro))
+        print(f"{name} relative L2 after {n} steps:", {k: f"{v:.2e}" for k, v in errs.items()},
+              f"max per-step mass difference {drift:.2e}", flush=True)
+        if prec == _lib.DGBGK_FP64:
+            for k, v in errs.items():
+                assert v < FP64_TOL, (k, v)
+            assert drift < MASS_TOL
+        else:
+            for k in ("rho", "temp", "ps"):
+                assert errs[k] < 1e-3, (k, errs[k])
+            assert errs["u,v"] < 5e-3
+            assert drift < 1e-3
+        s.close()
