@@ -510,30 +510,56 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
 // n, n*u, n*v; pass 2: pressure and mean relative speed about u.  Writes the
 // FP64 node fields and the next step's GTEQNF/COMPMU parameters in T.
 // ---------------------------------------------------------------------------
+// k_moments geometry: a block of 256 threads serves 256 / S nodes with S
+// wavefronts per node, each summing its share of the ordinates; the shares
+// combine through LDS in wavefront order.  Few nodes give few wavefronts
+// (one per 64 nodes at S = 1), too few to hide FP64 latency; many nodes
+// keep the chip busy at S = 1, where the combine would only cost.
+// kMomRing: register sets of four ordinates each wavefront keeps in its
+// load ring.
+#ifndef DGBGK_MOM_RING32
+#define DGBGK_MOM_RING32 2
+#endif
+#ifndef DGBGK_MOM_RING64
+#define DGBGK_MOM_RING64 2
+#endif
 template <typename T>
+constexpr int kMomRing = sizeof(T) == 4 ? DGBGK_MOM_RING32 : DGBGK_MOM_RING64;
+
+template <typename T, int S>
 __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
   constexpr bool kComp = sizeof(T) < sizeof(double);
+  constexpr int kMomNodes = 256 / S;
   __shared__ double s_red[8];
+  __shared__ double s_part[S > 1 ? S : 1][6][S > 1 ? kMomNodes : 1];
+  // the wavefront's share index, made a scalar: the ordinate loops then
+  // read the velocity tables with scalar loads
+  const int lane = threadIdx.x % kMomNodes;
+  const int sub = __builtin_amdgcn_readfirstlane(threadIdx.x / kMomNodes);
   const size_t N3 = 3 * (size_t)a.E;
-  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t k = (size_t)blockIdx.x * kMomNodes + lane;
+  const bool live = k < N3;
   const int V = a.V;
-  double mu = 0.0, mass = 0.0;
-  if (k < N3) {
-    const T* fp = a.f + k;
-    const double* cxa = a.vtd;
-    const double* cya = a.vtd + V;
-    const double* cxb = a.vtd + 2 * V;
-    const double* cyb = a.vtd + 3 * V;
-    const double* wv = a.vtd + 4 * V;
-    // First pass (n, n u, n v), summed per ordinate chunk of the update
-    // kernel and then over the chunks: plainly in the FP64 form (with one
-    // chunk that is the reference's ordinate order), with Kahan compensation
-    // over the chunk sums in the FP32-bulk form.  Inside a step k_update
-    // has left the chunk sums in msum (wall elements excepted), computed
-    // in exactly this order.
-    double s0 = 0, c0 = 0, s1 = 0, c1 = 0, s2 = 0, c2 = 0;
+  const T* fp = a.f + (live ? k : 0);
+  const double* cxa = a.vtd;
+  const double* cya = a.vtd + V;
+  const double* cxb = a.vtd + 2 * V;
+  const double* cyb = a.vtd + 3 * V;
+  const double* wv = a.vtd + 4 * V;
+
+  // First pass (n, n u, n v), summed per ordinate chunk of the update
+  // kernel, then over the chunks: this wavefront's contiguous share of the
+  // chunks, then the shares in wavefront order.  Plain sums in the FP64
+  // form (one chunk and one share give the reference's ordinate order);
+  // Kahan compensation over chunk sums and shares in the FP32-bulk form.
+  // Inside a step k_update has left the chunk sums in msum (wall elements
+  // excepted), computed in exactly this order.
+  double s0 = 0, c0 = 0, s1 = 0, c1 = 0, s2 = 0, c2 = 0;
+  if (live) {
     const bool have = a.msum != nullptr && a.ewslot[k / 3] < 0;
-    for (int ch = 0; ch < a.nchunks; ++ch) {
+    const int cper = (a.nchunks + S - 1) / S;
+    const int cb = min(a.nchunks, sub * cper), ce = min(a.nchunks, cb + cper);
+    for (int ch = cb; ch < ce; ++ch) {
       double b0 = 0, b1 = 0, b2 = 0;
       if (have) {
         const double* ms = a.msum + (size_t)ch * 3 * N3 + k;
@@ -566,64 +592,107 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
       nsum<kComp>(s1, c1, b1);
       nsum<kComp>(s2, c2, b2);
     }
-    const double n = s0 - c0;
-    double nd = 0.0, ux = 0.0, uy = 0.0;
-    if (n >= 1e-20) {
-      nd = n;
-      ux = (s1 - c1) / n;
-      uy = (s2 - c2) / n;
-    }
-    double sp = 0, cp = 0, st = 0, ct = 0;
-    const bool dens = n >= 1e-20;
-    // second pass, four ordinates at a time
-    auto block4 = [&](const T (&raw)[4], int v) {
-      double bp = 0, bt = 0;
+  }
+  double t0 = s0, e0 = c0, t1 = s1, e1 = c1, t2 = s2, e2s = c2;
+  if constexpr (S > 1) {
+    s_part[sub][0][lane] = s0;
+    s_part[sub][1][lane] = c0;
+    s_part[sub][2][lane] = s1;
+    s_part[sub][3][lane] = c1;
+    s_part[sub][4][lane] = s2;
+    s_part[sub][5][lane] = c2;
+    __syncthreads();
+    // every wavefront combines the shares in the same order, so all hold
+    // the same n, u, v
+    t0 = e0 = t1 = e1 = t2 = e2s = 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const double fw = (double)raw[j] * ldc(wv, v + j);
-        const double dx = ldc(cxb, v + j) - ux, dy = ldc(cyb, v + j) - uy;
-        const double ex = ux - ldc(cxa, v + j), ey = uy - ldc(cya, v + j);
-        const double e2 = ex * ex + ey * ey;
-        // FP32-bulk: the hardware square root (single-precision accuracy,
-        // far below the FP32 rounding of f); FP64: correctly rounded
-        const double rel = kComp ? __builtin_amdgcn_sqrt(e2) : sqrt(e2);
-        if (kComp) {
-          bp += 0.5 * (dx * dx + dy * dy) * fw;
-          bt += rel * fw;
-        } else {
-          sp += 0.5 * (dx * dx + dy * dy) * fw;
-          if (dens) st += rel * fw;
+    for (int i = 0; i < S; ++i) {
+      nsum<kComp>(t0, e0, s_part[i][0][lane] - s_part[i][1][lane]);
+      nsum<kComp>(t1, e1, s_part[i][2][lane] - s_part[i][3][lane]);
+      nsum<kComp>(t2, e2s, s_part[i][4][lane] - s_part[i][5][lane]);
+    }
+  }
+  const double n = t0 - e0;
+  double nd = 0.0, ux = 0.0, uy = 0.0;
+  if (n >= 1e-20) {
+    nd = n;
+    ux = (t1 - e1) / n;
+    uy = (t2 - e2s) / n;
+  }
+  const bool dens = n >= 1e-20;
+
+  // Second pass (pressure and COMPMU's mean relative speed) over this
+  // wavefront's contiguous share of the ordinates, four at a time.
+  double sp = 0, cp = 0, st = 0, ct = 0;
+  auto block4 = [&](const T (&raw)[4], int v) {
+    double bp = 0, bt = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double fw = (double)raw[j] * ldc(wv, v + j);
+      const double dx = ldc(cxb, v + j) - ux, dy = ldc(cyb, v + j) - uy;
+      const double ex = ux - ldc(cxa, v + j), ey = uy - ldc(cya, v + j);
+      const double e2 = ex * ex + ey * ey;
+      // FP32-bulk: the hardware square root (single-precision accuracy,
+      // far below the FP32 rounding of f); FP64: correctly rounded
+      const double rel = kComp ? __builtin_amdgcn_sqrt(e2) : sqrt(e2);
+      if (kComp) {
+        bp += 0.5 * (dx * dx + dy * dy) * fw;
+        bt += rel * fw;
+      } else {
+        sp += 0.5 * (dx * dx + dy * dy) * fw;
+        if (dens) st += rel * fw;
+      }
+    }
+    if (kComp) {
+      nsum<true>(sp, cp, bp);
+      if (dens) nsum<true>(st, ct, bt);
+    }
+  };
+  const int vper = ((V + S - 1) / S + 3) / 4 * 4;
+  const int vb = min(V, sub * vper), ve = min(V, vb + vper);
+  int v = vb;
+  if (live) {
+    // A ring of kMomRing register sets of four ordinates: while one set is
+    // summed, the loads of the next sets are in flight.  Ring positions are
+    // compile-time (unrolled), loads are clamped to the share's last
+    // ordinate (none is conditional), and the sums run in ordinate order.
+    constexpr int kR = kMomRing<T>;
+    if (ve - vb >= 4 * kR) {
+      T ring[kR][4];
+#pragma unroll
+      for (int r = 0; r < kR; ++r)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ring[r][j] = fp[(size_t)(vb + 4 * r + j) * N3];
+      for (; v + 4 * kR <= ve; v += 4 * kR) {
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+          T cur[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            cur[j] = ring[r][j];
+            ring[r][j] = fp[(size_t)min(v + 4 * (kR + r) + j, ve - 1) * N3];
+          }
+          block4(cur, v + 4 * r);
         }
       }
-      if (kComp) {
-        nsum<true>(sp, cp, bp);
-        if (dens) nsum<true>(st, ct, bt);
-      }
-    };
-    // ping-pong over two register sets: the next four ordinates' loads are
-    // in flight while the current four are summed (loads clamped to the
-    // last ordinate, so none is conditional)
-    int v = 0;
-    if (V >= 8) {
-      T ra[4], rb[4];
+      // fewer than 4 kR ordinates remain; the ring holds them in order
 #pragma unroll
-      for (int j = 0; j < 4; ++j) ra[j] = fp[(size_t)j * N3];
-      for (; v + 8 <= V; v += 8) {
+      for (int r = 0; r < kR; ++r)
+        if (v + 4 <= ve) {
+          T cur[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) rb[j] = fp[(size_t)(v + 4 + j) * N3];
-        block4(ra, v);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) ra[j] = fp[(size_t)min(v + 8 + j, V - 1) * N3];
-        block4(rb, v + 4);
-      }
+          for (int j = 0; j < 4; ++j) cur[j] = ring[r][j];
+          block4(cur, v);
+          v += 4;
+        }
     }
-    for (; v + 4 <= V; v += 4) {
+    for (; v + 4 <= ve; v += 4) {
       T rc[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) rc[j] = fp[(size_t)(v + j) * N3];
       block4(rc, v);
     }
-    for (; v < V; ++v) {
+    for (; v < ve; ++v) {
       const double fw = (double)fp[(size_t)v * N3] * ldc(wv, v);
       const double dx = ldc(cxb, v) - ux, dy = ldc(cyb, v) - uy;
       nsum<kComp>(sp, cp, 0.5 * (dx * dx + dy * dy) * fw);
@@ -633,8 +702,26 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
         nsum<kComp>(st, ct, (kComp ? __builtin_amdgcn_sqrt(e2) : sqrt(e2)) * fw);
       }
     }
-    const double ps = (sp - cp) * a.mm;
-    const double ctv = dens ? (st - ct) / n : 0.0;
+  }
+  double tp = sp, ep = cp, tt = st, et = ct;
+  if constexpr (S > 1) {
+    __syncthreads();  // every wavefront has read the first-pass shares
+    s_part[sub][0][lane] = sp;
+    s_part[sub][1][lane] = cp;
+    s_part[sub][2][lane] = st;
+    s_part[sub][3][lane] = ct;
+    __syncthreads();
+    tp = ep = tt = et = 0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      nsum<kComp>(tp, ep, s_part[i][0][lane] - s_part[i][1][lane]);
+      nsum<kComp>(tt, et, s_part[i][2][lane] - s_part[i][3][lane]);
+    }
+  }
+  double mu = 0.0, mass = 0.0;
+  if (sub == 0 && live) {
+    const double ps = (tp - ep) * a.mm;
+    const double ctv = dens ? (tt - et) / n : 0.0;
     mu = sqrt(2.0) * ctv * a.sigt * nd;
     a.mom[k] = nd;
     a.mom[N3 + k] = ux;
@@ -899,6 +986,7 @@ struct Ctx {
   int slots = 256;   // update kernel: tile slots (threads per block)
   int depth = 1;     // update kernel: ordinates of f in flight (1 or 2)
   int fuse_mom = 1;  // update kernel also does the first moment pass
+  int mom_split = 1; // k_moments wavefronts per node (1, 2 or 4)
   int ntiles = 0, vchunk = 1, nvchunks = 1, nblk_upd = 0, nblk_wall = 0, ngroups = 1,
       nblk_mom = 0, neta = 0, nforce = 0;
   int itime = 0;
@@ -937,7 +1025,12 @@ int launch_moments(Ctx& c, bool partials = false) {
   m.vchunk = c.vchunk;
   m.E = c.p.E;
   m.V = c.p.V;
-  hipLaunchKernelGGL(k_moments<T>, dim3(c.nblk_mom), dim3(256), 0, c.stream, m);
+  if (c.mom_split == 4)
+    hipLaunchKernelGGL((k_moments<T, 4>), dim3(c.nblk_mom), dim3(256), 0, c.stream, m);
+  else if (c.mom_split == 2)
+    hipLaunchKernelGGL((k_moments<T, 2>), dim3(c.nblk_mom), dim3(256), 0, c.stream, m);
+  else
+    hipLaunchKernelGGL((k_moments<T, 1>), dim3(c.nblk_mom), dim3(256), 0, c.stream, m);
   HIPCK(hipGetLastError());
   return kOk;
 }
@@ -1439,7 +1532,20 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
     c.gsize = p.V;
   }
   c.nblk_wall = nwe ? (nwe * c.ngroups + c.wall_tpb - 1) / c.wall_tpb : 0;
-  c.nblk_mom = (int)((3 * (size_t)p.E + 255) / 256);
+  // k_moments: one wavefront per 64 nodes leaves fewer than ~8 per SIMD
+  // below 512k nodes, and then up to four share each node's ordinates.  The
+  // choice depends on the node count only, so a mesh gives the same sums
+  // (and results) on any device.
+  {
+    const size_t n3 = 3 * (size_t)p.E;
+    c.mom_split = n3 < (size_t)1 << 19 ? 4 : (n3 < (size_t)1 << 20 ? 2 : 1);
+    if (const char* ms = std::getenv("DGBGK_MOM_SPLIT")) {
+      const int v = std::atoi(ms);
+      if (v == 1 || v == 2 || v == 4) c.mom_split = v;
+    }
+    const size_t per = 256 / c.mom_split;
+    c.nblk_mom = (int)((n3 + per - 1) / per);
+  }
   c.neta = (int)p.eta_side.size();
   c.nforce = (int)p.force_p1.size();
   st = c.prec ? upload_tables<float>(c) : upload_tables<double>(c);
