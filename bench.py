@@ -105,6 +105,8 @@ def main():
     dofv = 3 * E * V
     print(f"# workload {args.config} E={E} V={V} ({case.rorder}x{case.torder}) "
           f"{args.precision} setup {setup_s:.1f}s", file=sys.stderr, flush=True)
+    print("# launch " + " ".join(f"{k}={v}" for k, v in s.launch_config().items()),
+          file=sys.stderr, flush=True)
 
     s.step(args.warmup, records=False)
     s.sync()

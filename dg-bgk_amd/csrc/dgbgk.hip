@@ -129,6 +129,7 @@ struct SlotState {
   T gx[3], gy[3], pA[3], pB[3], pU[3], pV[3], pMd[3];  // phase A (all slots)
   T gxa[3], gya[3], mdt, nx[3], ny[3], al[3];          // phase B (owned slots)
   int info[3], mask, wsl, e;
+  int nb1[3], nb2[3];  // LDS index (4 slot + node) of the neighbour's values at k, k1
   bool owned, active;
 };
 
@@ -153,7 +154,7 @@ template <typename T, bool kMom>
 __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotState<T>& s,
                                                 int v, T ux, T uy, T fin, const T (&f)[3],
                                                 T* __restrict__ sf, T dte, T& res, T& chg,
-                                                double wv, double cx, double cy,
+                                                double wv, double wcx, double wcy,
                                                 double (&m0)[3], double (&m1)[3],
                                                 double (&m2)[3]) {
   const int slot = threadIdx.x;
@@ -200,11 +201,10 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
     const int in = s.info[k];
     const T cn = ux * s.nx[k] + uy * s.ny[k];
     const bool own = (in & kUpdOwn) || cn > T(0) || ((in & kUpdRight) && cn == T(0));
-    const int src = own ? slot : (in >> kUpdSlot);
-    const int j1 = own ? k : ((in >> kUpdNbK) & 3);
-    const int j2 = own ? k1 : ((in >> kUpdNbK1) & 3);
-    const T* q = sf + 4 * src;
-    const T g1 = q[j1], g2 = q[j2];
+    // the neighbour's values are read whichever side is upwind (fixed LDS
+    // indices, no address arithmetic); the element's own are in registers
+    const T n1 = sf[s.nb1[k]], n2 = sf[s.nb2[k]];
+    const T g1 = own ? h[k] : n1, g2 = own ? h[k1] : n2;
     const T w = -(s.al[k] * cn);
     rhs[k] += w * (T(2) * g1 + g2);
     rhs[k1] += w * (g1 + T(2) * g2);
@@ -225,7 +225,7 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
     fn[k] = f[k] + d[k];
     fn[k] = fn[k] < T(0) ? T(0) : fn[k];  // NaN stays NaN, as in the reference
   }
-  if (s.mask) {  // inflow nodes: rare, so the selects sit behind a branch
+  if (__ballot(s.mask != 0)) {  // inflow nodes: rare; a wave-uniform branch
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       if ((s.mask >> k) & 1) fn[k] = fin;
@@ -237,11 +237,12 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
     fo[k] = fn[k];
     if (kMom) {
       // MACROS' first pass (src/macros.f90:93-108) on f^{k+1}, this chunk's
-      // ordinates, in FP64
-      const double fw = (double)fn[k] * wv;
-      m0[k] += fw;
-      m1[k] += cx * fw;
-      m2[k] += cy * fw;
+      // ordinates, in FP64 (wcx = w cx, wcy = w cy; k_moments sums the same
+      // products in the same order)
+      const double fd = (double)fn[k];
+      m0[k] = fma(wv, fd, m0[k]);
+      m1[k] = fma(wcx, fd, m1[k]);
+      m2[k] = fma(wcy, fd, m2[k]);
     }
   }
 }
@@ -264,7 +265,9 @@ template <typename T, int kDepth, bool kMom, int kS>
 __global__ void __launch_bounds__(kS) __attribute__((amdgpu_waves_per_eu(kMinWaves<T>, 8)))
 k_update(UpdArgs<T> a) {
   constexpr int kQ = kDepth + 1;
-  __shared__ __align__(16) T s_f[2][kS * 4];
+  // one LDS buffer per queue position: ordinate v + kQ reuses v's buffer
+  // after kQ - 1 >= 1 barriers, and the buffer is a compile-time offset
+  __shared__ __align__(16) T s_f[kQ][kS * 4];
   __shared__ double s_red[8];
   const int logical = xcd_logical(blockIdx.x, gridDim.x);
   const int tile = logical % a.ntiles;
@@ -287,6 +290,8 @@ k_update(UpdArgs<T> a) {
     s.gx[k] = s.gy[k] = s.pA[k] = s.pB[k] = s.pU[k] = s.pV[k] = s.pMd[k] = T(0);
     s.gxa[k] = s.gya[k] = s.nx[k] = s.ny[k] = s.al[k] = T(0);
     s.info[k] = kUpdOwn;
+    s.nb1[k] = 4 * slot + k;
+    s.nb2[k] = 4 * slot + (k + 1) % 3;
   }
   s.mdt = T(0);
   s.mask = 0;
@@ -312,6 +317,11 @@ k_update(UpdArgs<T> a) {
       s.gxa[k] = s.gx[k] * area;
       s.gya[k] = s.gy[k] * area;
       s.info[k] = a.uinfo[k * E + e];
+      if (!(s.info[k] & kUpdOwn)) {  // interior side: the neighbour's slot and nodes
+        const int nbs = s.info[k] >> kUpdSlot;
+        s.nb1[k] = 4 * nbs + ((s.info[k] >> kUpdNbK) & 3);
+        s.nb2[k] = 4 * nbs + ((s.info[k] >> kUpdNbK1) & 3);
+      }
       s.nx[k] = a.unx[k * E + e];
       s.ny[k] = a.uny[k * E + e];
       s.al[k] = a.ual[k * E + e];
@@ -341,8 +351,8 @@ k_update(UpdArgs<T> a) {
       qi[j] = ldc(a.vtab, 2 * V + v);
       if (kMom) {
         qw[j] = ldc(a.vtd, 4 * V + v);
-        qcx[j] = ldc(a.vtd, 2 * V + v);
-        qcy[j] = ldc(a.vtd, 3 * V + v);
+        qcx[j] = ldc(a.vtd, 5 * V + v);
+        qcy[j] = ldc(a.vtd, 6 * V + v);
       }
     }
   }
@@ -366,11 +376,11 @@ k_update(UpdArgs<T> a) {
           qi[jn] = ldc(a.vtab, 2 * V + vn);
           if (kMom) {
             qw[jn] = ldc(a.vtd, 4 * V + vn);
-            qcx[jn] = ldc(a.vtd, 2 * V + vn);
-            qcy[jn] = ldc(a.vtd, 3 * V + vn);
+            qcx[jn] = ldc(a.vtd, 5 * V + vn);
+            qcy[jn] = ldc(a.vtd, 6 * V + vn);
           }
         }
-        update_ordinate<T, kMom>(a, s, v, qx[j], qy[j], qi[j], q[j], s_f[(v - v0) & 1], dte,
+        update_ordinate<T, kMom>(a, s, v, qx[j], qy[j], qi[j], q[j], s_f[j], dte,
                                  res, chg, qw[j], qcx[j], qcy[j], m0, m1, m2);
       }
     }
@@ -546,6 +556,8 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
   const double* cxb = a.vtd + 2 * V;
   const double* cyb = a.vtd + 3 * V;
   const double* wv = a.vtd + 4 * V;
+  const double* wcx = a.vtd + 5 * V;
+  const double* wcy = a.vtd + 6 * V;
 
   // First pass (n, n u, n v), summed per ordinate chunk of the update
   // kernel, then over the chunks: this wavefront's contiguous share of the
@@ -575,17 +587,16 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
           for (int j = 0; j < 4; ++j) fv[j] = (double)fp[(size_t)(v + j) * N3];
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            const double fw = fv[j] * ldc(wv, v + j);
-            b0 += fw;
-            b1 += ldc(cxb, v + j) * fw;
-            b2 += ldc(cyb, v + j) * fw;
+            b0 = fma(ldc(wv, v + j), fv[j], b0);
+            b1 = fma(ldc(wcx, v + j), fv[j], b1);
+            b2 = fma(ldc(wcy, v + j), fv[j], b2);
           }
         }
         for (; v < ve; ++v) {
-          const double fw = (double)fp[(size_t)v * N3] * ldc(wv, v);
-          b0 += fw;
-          b1 += ldc(cxb, v) * fw;
-          b2 += ldc(cyb, v) * fw;
+          const double fd = (double)fp[(size_t)v * N3];
+          b0 = fma(ldc(wv, v), fd, b0);
+          b1 = fma(ldc(wcx, v), fd, b1);
+          b2 = fma(ldc(wcy, v), fd, b2);
         }
       }
       nsum<kComp>(s0, c0, b0);
@@ -1100,6 +1111,27 @@ void launch_update_s(Ctx& c, const UpdArgs<T>& u) {
   }
 }
 
+// Resident blocks per CU of the update instantiation c will launch.
+template <typename T, int kS>
+int update_blocks_per_cu_s(const Ctx& c) {
+  int n = 0;
+  hipError_t e;
+  if (c.fuse_mom)
+    e = c.depth == 2
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 2, true, kS>, kS, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 1, true, kS>, kS, 0);
+  else
+    e = c.depth == 2
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 2, false, kS>, kS, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 1, false, kS>, kS, 0);
+  return e == hipSuccess && n > 0 ? n : 1;
+}
+
+template <typename T>
+int update_blocks_per_cu(const Ctx& c) {
+  return c.slots == 512 ? update_blocks_per_cu_s<T, 512>(c) : update_blocks_per_cu_s<T, 256>(c);
+}
+
 template <typename T>
 void launch_update(Ctx& c, const UpdArgs<T>& u) {
   if (c.slots == 512)
@@ -1267,7 +1299,7 @@ int upload_tables(Ctx& c) {
   if ((st = upload((T**)&c.ual, ual, c.owned))) return st;
   if ((st = upload(&c.uinfo, uinfo, c.owned))) return st;
   std::vector<T> vtab(3 * (size_t)V);
-  std::vector<double> vtd(5 * (size_t)V);
+  std::vector<double> vtd(7 * (size_t)V);
   for (int v = 0; v < V; ++v) {
     vtab[v] = (T)p.cxa[v];
     vtab[V + v] = (T)p.cya[v];
@@ -1277,6 +1309,9 @@ int upload_tables(Ctx& c) {
     vtd[2 * V + v] = p.cxb[v];
     vtd[3 * V + v] = p.cyb[v];
     vtd[4 * V + v] = p.wv[v];
+    // the first moment pass's weights: n u = sum (w cx) f, n v = sum (w cy) f
+    vtd[5 * V + v] = p.wv[v] * p.cxb[v];
+    vtd[6 * V + v] = p.wv[v] * p.cyb[v];
   }
   if ((st = upload((T**)&c.vtab, vtab, c.owned))) return st;
   if ((st = upload(&c.vtd, vtd, c.owned))) return st;
@@ -1501,9 +1536,29 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   c.ntiles = (int)p.tile_e0.size();
   int vch = prm->vchunk;
   if (vch <= 0) {
-    const int want = 4096;  // blocks: 16 per CU on 256 CUs
-    int nv = std::max(1, std::min(p.V, (want + c.ntiles - 1) / c.ntiles));
-    vch = (p.V + nv - 1) / nv;
+    // Ordinate chunks per tile.  Each chunk re-reads its slots' tables
+    // (Maxwellian parameters, geometry, edges: ~160 B in FP32, ~300 B in
+    // FP64 per slot), so fewer is better, as long as the blocks fill the
+    // chip's resident capacity and do not leave a short last wave: cost =
+    // (capacity rounds used / rounds of work) x (1 + table share x chunks).
+    // The capacity takes 256 CUs (MI355X) whatever the device, so the
+    // chunking, and with it the FP64 summation order, depends on the mesh
+    // only.
+    const int bpc = c.prec ? update_blocks_per_cu<float>(c) : update_blocks_per_cu<double>(c);
+    const double cap = 256.0 * bpc;
+    const double tab = (c.prec ? 160.0 : 300.0) / (2.0 * (c.prec ? 4 : 8) * 3.0 * p.V);
+    double best = 1e300;
+    int bnv = 1;
+    for (int nv = 1; nv <= std::min(p.V, 64); ++nv) {
+      const double B = (double)c.ntiles * nv;
+      const double eff = std::max(cap / B, std::ceil(B / cap) * cap / B);
+      const double cost = eff * (1.0 + tab * nv);
+      if (cost < best * (1.0 - 1e-9)) {
+        best = cost;
+        bnv = nv;
+      }
+    }
+    vch = (p.V + bnv - 1) / bnv;
   }
   c.vchunk = std::max(1, std::min(vch, p.V));
   c.nvchunks = (p.V + c.vchunk - 1) / c.vchunk;
@@ -1805,6 +1860,19 @@ int dgbgk_sync(dgbgk_ctx* h) {
 int dgbgk_dof_bytes(dgbgk_ctx* h, int* bytes_per_value) {
   if (!h) return fail(kErrArg, "null context");
   if (bytes_per_value) *bytes_per_value = (int)h->c.tsz;
+  return kOk;
+}
+
+int dgbgk_launch_config(dgbgk_ctx* h, int* slots, int* depth, int* vchunk, int* nvchunks,
+                        int* ntiles, int* mom_split) {
+  if (!h) return fail(kErrArg, "null context");
+  const Ctx& c = h->c;
+  if (slots) *slots = c.slots;
+  if (depth) *depth = c.depth;
+  if (vchunk) *vchunk = c.vchunk;
+  if (nvchunks) *nvchunks = c.nvchunks;
+  if (ntiles) *ntiles = c.ntiles;
+  if (mom_split) *mom_split = c.mom_split;
   return kOk;
 }
 

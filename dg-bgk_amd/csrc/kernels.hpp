@@ -54,7 +54,7 @@ struct UpdArgs {
   const int* __restrict__ tiles;   // [ntiles][4] e0, nown, hoff, nhalo
   const int* __restrict__ halo;
   T* __restrict__ wpart;           // [NWE][V][8] partial RHS (3), COLL (3), U
-  const double* __restrict__ vtd;  // [5][V]: cxa, cya, cxb, cyb, wv
+  const double* __restrict__ vtd;  // [7][V]: cxa, cya, cxb, cyb, wv, wv cxb, wv cyb
   double* __restrict__ msum;       // [nvchunks][3][3E] moment partials of f^{k+1}
   const double* __restrict__ scal; // [0] = DTE
   double* __restrict__ res_part;   // per block: max|increment|; +res_stride: max|change|
@@ -93,7 +93,7 @@ struct WallArgs {
 template <typename T>
 struct MomArgs {
   const T* __restrict__ f;
-  const double* __restrict__ vtd;  // [5][V]: cxa, cya, cxb, cyb, wv
+  const double* __restrict__ vtd;  // [7][V]: cxa, cya, cxb, cyb, wv, wv cxb, wv cyb
   const double* __restrict__ rj;   // GEOME(7,e)
   double* __restrict__ mom;        // [5][3E]
   T* __restrict__ np;              // [5][3E]

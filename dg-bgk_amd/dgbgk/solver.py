@@ -190,3 +190,9 @@ class Solver:
         b = C.c_int()
         check(lib().dgbgk_dof_bytes(self.h, C.byref(b)))
         return b.value
+
+    def launch_config(self) -> dict:
+        v = [C.c_int() for _ in range(6)]
+        check(lib().dgbgk_launch_config(self.h, *(C.byref(x) for x in v)))
+        return dict(zip(("slots", "depth", "vchunk", "nvchunks", "ntiles", "mom_split"),
+                        (x.value for x in v)))

@@ -157,6 +157,13 @@ int dgbgk_sync(dgbgk_ctx *ctx);
  * update of the update kernel (one read and one write of f). */
 int dgbgk_dof_bytes(dgbgk_ctx *ctx, int *bytes_per_value);
 
+/* Launch configuration the library chose for this mesh (no counterpart in
+ * the reference, which has no device launch): update tile slots, f
+ * ordinates in flight, ordinates per update block and chunks, tiles, and
+ * the moments kernel's wavefronts per node. */
+int dgbgk_launch_config(dgbgk_ctx *ctx, int *slots, int *depth, int *vchunk, int *nvchunks,
+                        int *ntiles, int *mom_split);
+
 /* ---- learned warm start: coordinate MLP (no counterpart in the reference) ----
  * fin -> width (SiLU), (hidden-1) x width -> width (SiLU), width -> fout.
  * BF16 weights and activations, FP32 accumulation (MFMA) and FP32 master
