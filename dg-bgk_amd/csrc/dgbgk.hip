@@ -217,7 +217,7 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
   }
   // GETINC + ADTHEM + INFLOW (src/getinc.f90:30-42, src/adthem.f90:109-118,
   // src/inflow.f90:165-181); mdt = 6/RJ * DTE
-  T* fo = a.fnxt + (size_t)v * N3 + 3 * (size_t)s.e;
+  T* fo = a.fnxt + (size_t)v * a.ldf + 3 * (size_t)s.e;
   T d[3], fn[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -342,7 +342,7 @@ k_update(UpdArgs<T> a) {
     qw[j] = qcx[j] = qcy[j] = 0.0;
     if (j < kDepth) {  // slots without an element read element 0
       const int v = min(v0 + j, v1 - 1);
-      const T* fp = fe + (size_t)v * N3;
+      const T* fp = fe + (size_t)v * a.ldf;
       q[j][0] = fp[0];
       q[j][1] = fp[1];
       q[j][2] = fp[2];
@@ -367,7 +367,7 @@ k_update(UpdArgs<T> a) {
         const int vn = min(v + kDepth, v1 - 1);
         const int jn = (j + kDepth) % kQ;
         {
-          const T* fp = fe + (size_t)vn * N3;
+          const T* fp = fe + (size_t)vn * a.ldf;
           q[jn][0] = fp[0];
           q[jn][1] = fp[1];
           q[jn][2] = fp[2];
@@ -449,7 +449,7 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
       const T cold[3] = {wp[3], wp[4], wp[5]};  // collision term times DTE
       const T U = wp[6];
       const T ux = a.vtab[v], uy = a.vtab[V + v];
-      const T* fc = a.fcur + (size_t)v * N3 + 3 * (size_t)e;
+      const T* fc = a.fcur + (size_t)v * a.ldf + 3 * (size_t)e;
       const T f[3] = {fc[0], fc[1], fc[2]};
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -473,14 +473,14 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
           const T* s1;
           const T* s2;
           if (use_lds) {
-            s1 = i1 < v ? s_ring + 3 * (i1 - vb) : a.fcur + (size_t)i1 * N3 + 3 * (size_t)e;
-            s2 = i2 < v ? s_ring + 3 * (i2 - vb) : a.fcur + (size_t)i2 * N3 + 3 * (size_t)e;
+            s1 = i1 < v ? s_ring + 3 * (i1 - vb) : a.fcur + (size_t)i1 * a.ldf + 3 * (size_t)e;
+            s2 = i2 < v ? s_ring + 3 * (i2 - vb) : a.fcur + (size_t)i2 * a.ldf + 3 * (size_t)e;
           } else if (a.jacobi) {
-            s1 = a.fcur + (size_t)i1 * N3 + 3 * (size_t)e;
-            s2 = a.fcur + (size_t)i2 * N3 + 3 * (size_t)e;
+            s1 = a.fcur + (size_t)i1 * a.ldf + 3 * (size_t)e;
+            s2 = a.fcur + (size_t)i2 * a.ldf + 3 * (size_t)e;
           } else {
-            s1 = (i1 < v ? a.fnxt : a.fcur) + (size_t)i1 * N3 + 3 * (size_t)e;
-            s2 = (i2 < v ? a.fnxt : a.fcur) + (size_t)i2 * N3 + 3 * (size_t)e;
+            s1 = (i1 < v ? a.fnxt : a.fcur) + (size_t)i1 * a.ldf + 3 * (size_t)e;
+            s2 = (i2 < v ? a.fnxt : a.fcur) + (size_t)i2 * a.ldf + 3 * (size_t)e;
           }
           const T rf1 = (zd2 * s1[a1] + zd1 * s2[a1]) / (zd1 + zd2);
           const T rf2 = (zd2 * s1[a2] + zd1 * s2[a2]) / (zd1 + zd2);
@@ -493,7 +493,7 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
         rhs[a1] += r1;
         rhs[a2] += r2;
       }
-      T* fo = a.fnxt + (size_t)v * N3 + 3 * (size_t)e;
+      T* fo = a.fnxt + (size_t)v * a.ldf + 3 * (size_t)e;
       const T fin = a.vtab[2 * V + v];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -584,7 +584,7 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
         for (; v + 4 <= ve; v += 4) {
           double fv[4];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) fv[j] = (double)fp[(size_t)(v + j) * N3];
+          for (int j = 0; j < 4; ++j) fv[j] = (double)fp[(size_t)(v + j) * a.ldf];
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             b0 = fma(ldc(wv, v + j), fv[j], b0);
@@ -593,7 +593,7 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
           }
         }
         for (; v < ve; ++v) {
-          const double fd = (double)fp[(size_t)v * N3];
+          const double fd = (double)fp[(size_t)v * a.ldf];
           b0 = fma(ldc(wv, v), fd, b0);
           b1 = fma(ldc(wcx, v), fd, b1);
           b2 = fma(ldc(wcy, v), fd, b2);
@@ -673,7 +673,7 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
 #pragma unroll
       for (int r = 0; r < kR; ++r)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) ring[r][j] = fp[(size_t)(vb + 4 * r + j) * N3];
+        for (int j = 0; j < 4; ++j) ring[r][j] = fp[(size_t)(vb + 4 * r + j) * a.ldf];
       for (; v + 4 * kR <= ve; v += 4 * kR) {
 #pragma unroll
         for (int r = 0; r < kR; ++r) {
@@ -681,7 +681,7 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             cur[j] = ring[r][j];
-            ring[r][j] = fp[(size_t)min(v + 4 * (kR + r) + j, ve - 1) * N3];
+            ring[r][j] = fp[(size_t)min(v + 4 * (kR + r) + j, ve - 1) * a.ldf];
           }
           block4(cur, v + 4 * r);
         }
@@ -700,11 +700,11 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
     for (; v + 4 <= ve; v += 4) {
       T rc[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) rc[j] = fp[(size_t)(v + j) * N3];
+      for (int j = 0; j < 4; ++j) rc[j] = fp[(size_t)(v + j) * a.ldf];
       block4(rc, v);
     }
     for (; v < ve; ++v) {
-      const double fw = (double)fp[(size_t)v * N3] * ldc(wv, v);
+      const double fw = (double)fp[(size_t)v * a.ldf] * ldc(wv, v);
       const double dx = ldc(cxb, v) - ux, dy = ldc(cyb, v) - uy;
       nsum<kComp>(sp, cp, 0.5 * (dx * dx + dy * dy) * fw);
       if (dens) {
@@ -764,7 +764,8 @@ __global__ void __launch_bounds__(256) k_eta(const T* __restrict__ f,
                                              const int* __restrict__ side,
                                              const double* __restrict__ geo,
                                              const double* __restrict__ vtd,
-                                             double* __restrict__ eta, int E, int V) {
+                                             double* __restrict__ eta, int E, int V,
+                                             long long ldf) {
   __shared__ double s_red[8];
   const int j = blockIdx.x;
   const int b = side[4 * j], e = side[4 * j + 1], in1 = side[4 * j + 2], in2 = side[4 * j + 3];
@@ -774,7 +775,7 @@ __global__ void __launch_bounds__(256) k_eta(const T* __restrict__ f,
   for (int v = threadIdx.x; v < V; v += blockDim.x) {
     const double test = vtd[2 * V + v] * anx + vtd[3 * V + v] * any;
     if (test > 0.0) {
-      const T* fp = f + (size_t)v * N3 + 3 * (size_t)e;
+      const T* fp = f + (size_t)v * ldf + 3 * (size_t)e;
       const double nf = 0.5 * ((double)fp[in1] + (double)fp[in2]);
       s += fabs(test) * nf * vtd[4 * V + v];
     }
@@ -897,7 +898,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_seed(T* __restrict__ f, const int* __restrict__ intma,
                                               const double* __restrict__ ps4,
                                               const double* __restrict__ vtd, int E, int V,
-                                              int P) {
+                                              int P, long long ldf) {
   const size_t N3 = 3 * (size_t)E;
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int v = blockIdx.y;
@@ -908,29 +909,31 @@ __global__ void __launch_bounds__(256) k_seed(T* __restrict__ f, const int* __re
   const double tmp = (ux - u0) * (ux - u0) + (uy - v0) * (uy - v0);
   const double sp = sqrt(tmp);
   const double f0 = (b2 / 3.1416) * exp(-(b2 * (sp * sp)));
-  f[(size_t)v * N3 + k] = (T)(n1 * f0);
+  f[(size_t)v * ldf + k] = (T)(n1 * f0);
 }
 
 // Layout conversions between the reference's DISNF(3,V,E) (double) and
 // the device's velocity-major T[V][3E].
 template <typename T>
-__global__ void k_to_device(const double* __restrict__ in, T* __restrict__ out, int E, int V) {
+__global__ void k_to_device(const double* __restrict__ in, T* __restrict__ out, int E, int V,
+                            long long ldf) {
   const size_t N3 = 3 * (size_t)E;
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int v = blockIdx.y;
   if (k >= N3) return;
   const size_t e = k / 3, a = k % 3;
-  out[(size_t)v * N3 + k] = (T)in[(e * V + v) * 3 + a];
+  out[(size_t)v * ldf + k] = (T)in[(e * V + v) * 3 + a];
 }
 
 template <typename T>
-__global__ void k_to_host(const T* __restrict__ in, double* __restrict__ out, int E, int V) {
+__global__ void k_to_host(const T* __restrict__ in, double* __restrict__ out, int E, int V,
+                          long long ldf) {
   const size_t N3 = 3 * (size_t)E;
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int v = blockIdx.y;
   if (k >= N3) return;
   const size_t e = k / 3, a = k % 3;
-  out[(e * V + v) * 3 + a] = (double)in[(size_t)v * N3 + k];
+  out[(e * V + v) * 3 + a] = (double)in[(size_t)v * ldf + k];
 }
 
 // ---------------------------------------------------------------------------
@@ -975,6 +978,7 @@ struct Ctx {
   int prec = 0;   // 0 fp64, 1 fp32
   int gs = 1;     // in-place sweep order
   size_t tsz = 8;
+  size_t ldf = 0;  // row stride of f (3E padded to 64 values)
   hipStream_t stream = nullptr;
   std::vector<void*> owned;
   void* f[2] = {nullptr, nullptr};
@@ -1035,6 +1039,7 @@ int launch_moments(Ctx& c, bool partials = false) {
   m.nchunks = c.nvchunks;
   m.vchunk = c.vchunk;
   m.E = c.p.E;
+  m.ldf = (long long)c.ldf;
   m.V = c.p.V;
   if (c.mom_split == 4)
     hipLaunchKernelGGL((k_moments<T, 4>), dim3(c.nblk_mom), dim3(256), 0, c.stream, m);
@@ -1148,7 +1153,7 @@ int step_once(Ctx& c) {
   if (c.neta) {
     tbegin(c, 3);
     hipLaunchKernelGGL(k_eta<T>, dim3(c.neta), dim3(256), 0, c.stream, fc, c.eta_side,
-                       c.eta_geo, c.vtd, c.eta, p.E, p.V);
+                       c.eta_geo, c.vtd, c.eta, p.E, p.V, (long long)c.ldf);
     HIPCK(hipGetLastError());
     tend(c, 3);
   }
@@ -1159,6 +1164,7 @@ int step_once(Ctx& c) {
   u.tiles = c.tiles; u.halo = c.halo; u.wpart = (T*)c.wpart; u.vtd = c.vtd; u.msum = c.msum;
   u.scal = c.scal; u.res_part = c.res_part; u.res_stride = c.nblk_upd + c.nblk_wall;
   u.E = p.E; u.V = p.V; u.ntiles = c.ntiles; u.vchunk = c.vchunk;
+  u.ldf = (long long)c.ldf;
   tbegin(c, 0);
   launch_update<T>(c, u);
   HIPCK(hipGetLastError());
@@ -1172,6 +1178,7 @@ int step_once(Ctx& c) {
     w.wall_elems = c.wall_elems; w.wpart = (const T*)c.wpart; w.scal = c.scal;
     w.res_part = c.res_part + c.nblk_upd; w.res_stride = c.nblk_upd + c.nblk_wall;
     w.alpha = (T)p.prm.alpha; w.E = p.E; w.V = p.V;
+    w.ldf = (long long)c.ldf;
     w.T_ = p.T; w.ngroups = c.ngroups; w.gsize = c.gsize; w.nwe = (int)p.wall_elems.size();
     w.jacobi = c.gs ? 0 : 1;
     w.diffuse = c.wall_diffuse;
@@ -1373,8 +1380,12 @@ int upload_tables(Ctx& c) {
   if ((st = alloc(&c.ps4, 4 * (size_t)p.P, c.owned))) return st;
   if ((st = alloc((T**)&c.wpart, std::max<size_t>(1, p.wall_elems.size() * (size_t)V * 8), c.owned)))
     return st;
+  // f rows padded to a multiple of 64 values: each row starts on a
+  // 256-byte boundary, so a wavefront's 64 consecutive nodes of one
+  // ordinate never straddle more cache lines than they fill
+  c.ldf = (N3 + 63) / 64 * 64;
   for (int b = 0; b < 2; ++b)
-    if ((st = alloc((T**)&c.f[b], (size_t)V * N3, c.owned))) return st;
+    if ((st = alloc((T**)&c.f[b], (size_t)V * c.ldf, c.owned))) return st;
   return kOk;
 }
 
@@ -1384,7 +1395,8 @@ int seed_points(Ctx& c, const std::vector<double>& ps4) {
                        c.stream));
   const size_t N3 = 3 * (size_t)c.p.E;
   hipLaunchKernelGGL(k_seed<T>, dim3((unsigned)((N3 + 255) / 256), c.p.V), dim3(256), 0,
-                     c.stream, (T*)c.f[c.cur], c.intma, c.ps4, c.vtd, c.p.E, c.p.V, c.p.P);
+                     c.stream, (T*)c.f[c.cur], c.intma, c.ps4, c.vtd, c.p.E, c.p.V, c.p.P,
+                     (long long)c.ldf);
   HIPCK(hipGetLastError());
   return kOk;
 }
@@ -1690,10 +1702,10 @@ int dgbgk_get_f(dgbgk_ctx* h, double* f) {
   dim3 g((unsigned)((N3 + 255) / 256), c.p.V);
   if (c.prec)
     hipLaunchKernelGGL(k_to_host<float>, g, dim3(256), 0, c.stream, (const float*)c.f[c.cur], d,
-                       c.p.E, c.p.V);
+                       c.p.E, c.p.V, (long long)c.ldf);
   else
     hipLaunchKernelGGL(k_to_host<double>, g, dim3(256), 0, c.stream,
-                       (const double*)c.f[c.cur], d, c.p.E, c.p.V);
+                       (const double*)c.f[c.cur], d, c.p.E, c.p.V, (long long)c.ldf);
   hipError_t e1 = hipMemcpyAsync(f, d, n * sizeof(double), hipMemcpyDeviceToHost, c.stream);
   hipError_t e2 = hipStreamSynchronize(c.stream);
   hipFree(d);
@@ -1712,10 +1724,10 @@ int dgbgk_set_f(dgbgk_ctx* h, const double* f) {
   dim3 g((unsigned)((N3 + 255) / 256), c.p.V);
   if (c.prec)
     hipLaunchKernelGGL(k_to_device<float>, g, dim3(256), 0, c.stream, d, (float*)c.f[c.cur],
-                       c.p.E, c.p.V);
+                       c.p.E, c.p.V, (long long)c.ldf);
   else
     hipLaunchKernelGGL(k_to_device<double>, g, dim3(256), 0, c.stream, d,
-                       (double*)c.f[c.cur], c.p.E, c.p.V);
+                       (double*)c.f[c.cur], c.p.E, c.p.V, (long long)c.ldf);
   int st = do_refresh(c);
   hipError_t e2 = hipStreamSynchronize(c.stream);
   hipFree(d);

@@ -1,7 +1,9 @@
 // kernels.hpp -- device kernels of the DG-BGK time march (gfx950).
 //
 // Data layout in HBM (one device, no partitioning):
-//   f        T[V][3E]   velocity-major; node (e,a) at 3e+a of each ordinate
+//   f        T[V][ldf]  velocity-major; node (e,a) at 3e+a of each ordinate;
+//                       rows padded to ldf = 3E rounded up to 64 values, so
+//                       every row starts on a 256-byte boundary
 //   geo      T[8][E]    dN/dx (3), dN/dy (3), area, 1/lumped mass  (SoA)
 //   einfo    int[3][E]  per element edge: kind | a1 | a2 | b1 | b2 | left
 //   eslot    int[3][E]  interior: neighbour's LDS slot; wall: wall-side slot
@@ -60,6 +62,7 @@ struct UpdArgs {
   double* __restrict__ res_part;   // per block: max|increment|; +res_stride: max|change|
   int res_stride;
   int E, V, ntiles, vchunk;
+  long long ldf;  // row stride of f
 };
 
 template <typename T>
@@ -88,6 +91,7 @@ struct WallArgs {
   int jacobi;    // reflections read f of the previous step only
   int diffuse;   // alpha = 1: the reflected part has weight 0 and is not read
   int lds_ring;  // one thread per speed ring, in-place reads staged in LDS
+  long long ldf;  // row stride of f
 };
 
 template <typename T>
@@ -103,6 +107,7 @@ struct MomArgs {
   const int* __restrict__ ewslot;   // wall elements have no partials
   double mm, sigt;
   int E, V, nchunks, vchunk;
+  long long ldf;  // row stride of f
 };
 
 }  // namespace dgbgk
