@@ -39,6 +39,31 @@ __device__ __forceinline__ int xcd_logical(int bid, int nblocks) {
   return xcd * q + min(xcd, r) + loc;
 }
 
+// A node's fixed dither phase in [0, 1): an integer hash of its index
+// (lowbias32 mixing), the same on every device and every run.
+__device__ __forceinline__ float node_phase(unsigned x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return (float)(x >> 8) * 0x1p-24f;
+}
+
+// Unbiased rounding of an FP32 store f + d: d gains a dither uniform over
+// +-1/2 ulp of f before the one rounding, so the stored value rounds up
+// with probability equal to the exact sum's fraction of an ulp.  Below
+// about an ulp per step a plain rounded store drops the increment, and a
+// run near steady state stalls at a state biased by ~ulp / (decay rate per
+// step).  The dither fraction r in [0, 1) runs through a Weyl sequence per
+// node (node, ordinate and step phases summed mod 1), equidistributed over
+// steps.  f = 0 and denormals have no exponent bits and get no dither, so
+// clamped nodes stay exactly zero.
+__device__ __forceinline__ float dither_inc(float f, float d, float r) {
+  const float u = __uint_as_float(__float_as_uint(f) & 0x7f800000u);  // 2^exponent(f)
+  return fmaf(fmaf(r, 0x1p-23f, -0x1p-24f), u, d);
+}
+
 __device__ __forceinline__ float dexp(float x) { return __expf(x); }
 __device__ __forceinline__ double dexp(double x) { return exp(x); }
 
@@ -130,6 +155,7 @@ struct SlotState {
   T gxa[3], gya[3], mdt, nx[3], ny[3], al[3];          // phase B (owned slots)
   int info[3], mask, wsl, e;
   int nb1[3], nb2[3];  // LDS index (4 slot + node) of the neighbour's values at k, k1
+  float dph[3];        // FP32 store dither: the nodes' fixed phases
   bool owned, active;
 };
 
@@ -152,7 +178,8 @@ __device__ __forceinline__ void store4<double>(double* p, double a, double b, do
 // block calls it for every v.
 template <typename T, bool kMom>
 __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotState<T>& s,
-                                                int v, T ux, T uy, T fin, const T (&f)[3],
+                                                int v, T ux, T uy, T fin, float ph,
+                                                const T (&f)[3],
                                                 T* __restrict__ sf, T dte, T& res, T& chg,
                                                 double wv, double wcx, double wcy,
                                                 double (&m0)[3], double (&m1)[3],
@@ -222,7 +249,11 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     d[k] = s.mdt * rhs[k] + cold[k];
-    fn[k] = f[k] + d[k];
+    T dk = d[k];
+    if constexpr (sizeof(T) == 4) {
+      if (a.dither) dk = dither_inc(f[k], dk, __builtin_amdgcn_fractf(s.dph[k] + ph));
+    }
+    fn[k] = f[k] + dk;
     fn[k] = fn[k] < T(0) ? T(0) : fn[k];  // NaN stays NaN, as in the reference
   }
   if (__ballot(s.mask != 0)) {  // inflow nodes: rare; a wave-uniform branch
@@ -231,7 +262,20 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
       if ((s.mask >> k) & 1) fn[k] = fin;
   }
   res = fmax(res, fmax(fmax(fabs(d[0]), fabs(d[1])), fabs(d[2])));
-  chg = fmax(chg, fmax(fmax(fabs(fn[0] - f[0]), fabs(fn[1] - f[1])), fabs(fn[2] - f[2])));
+  T cg[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) cg[k] = fabs(fn[k] - f[k]);
+  if constexpr (sizeof(T) == 4) {
+    // with the dithered store the change is the step's own increment, not
+    // the +-1 ulp its rounding adds: |d| where the store was neither
+    // clamped nor reset
+    if (a.dither) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (fn[k] != T(0) && !((s.mask >> k) & 1)) cg[k] = fabs(d[k]);
+    }
+  }
+  chg = fmax(chg, fmax(fmax(cg[0], cg[1]), cg[2]));
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     fo[k] = fn[k];
@@ -292,6 +336,7 @@ k_update(UpdArgs<T> a) {
     s.info[k] = kUpdOwn;
     s.nb1[k] = 4 * slot + k;
     s.nb2[k] = 4 * slot + (k + 1) % 3;
+    s.dph[k] = 0.f;
   }
   s.mdt = T(0);
   s.mask = 0;
@@ -328,17 +373,23 @@ k_update(UpdArgs<T> a) {
     }
     s.mask = a.emask[e];
     s.wsl = a.ewslot[e];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s.dph[k] = node_phase(3u * (unsigned)e + (unsigned)k);
   }
   T res = 0, chg = 0;
 
   // prime the queue: f and the ordinate's velocity and inflow value for
   // v0 .. v0+kDepth-1
   T q[kQ][3], qx[kQ], qy[kQ], qi[kQ];
+  float qp[kQ];  // FP32 store dither: the ordinate's phase plus this step's
   double qw[kQ], qcx[kQ], qcy[kQ];
+  const double stw = a.scal[6] * 0.6180339887498949;  // Weyl step phase
+  const float sph = (float)(stw - floor(stw));
   const T* fe = a.fcur + 3 * (size_t)e;
 #pragma unroll
   for (int j = 0; j < kQ; ++j) {
     q[j][0] = q[j][1] = q[j][2] = qx[j] = qy[j] = qi[j] = T(0);
+    qp[j] = 0.f;
     qw[j] = qcx[j] = qcy[j] = 0.0;
     if (j < kDepth) {  // slots without an element read element 0
       const int v = min(v0 + j, v1 - 1);
@@ -349,6 +400,7 @@ k_update(UpdArgs<T> a) {
       qx[j] = ldc(a.vtab, v);
       qy[j] = ldc(a.vtab, V + v);
       qi[j] = ldc(a.vtab, 2 * V + v);
+      qp[j] = sph + (float)ldc(a.vtab, 3 * V + v);
       if (kMom) {
         qw[j] = ldc(a.vtd, 4 * V + v);
         qcx[j] = ldc(a.vtd, 5 * V + v);
@@ -374,13 +426,14 @@ k_update(UpdArgs<T> a) {
           qx[jn] = ldc(a.vtab, vn);
           qy[jn] = ldc(a.vtab, V + vn);
           qi[jn] = ldc(a.vtab, 2 * V + vn);
+          qp[jn] = sph + (float)ldc(a.vtab, 3 * V + vn);
           if (kMom) {
             qw[jn] = ldc(a.vtd, 4 * V + vn);
             qcx[jn] = ldc(a.vtd, 5 * V + vn);
             qcy[jn] = ldc(a.vtd, 6 * V + vn);
           }
         }
-        update_ordinate<T, kMom>(a, s, v, qx[j], qy[j], qi[j], q[j], s_f[j], dte,
+        update_ordinate<T, kMom>(a, s, v, qx[j], qy[j], qi[j], qp[j], q[j], s_f[j], dte,
                                  res, chg, qw[j], qcx[j], qcy[j], m0, m1, m2);
       }
     }
@@ -443,6 +496,12 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
     }
     const int mask = a.emask[e];
     const T alpha = a.alpha;
+    // FP32 store dither (as in k_update): node, ordinate and step phases
+    float wph[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) wph[k] = node_phase(3u * (unsigned)e + (unsigned)k);
+    const double stw = a.scal[6] * 0.6180339887498949;
+    const float sph = (float)(stw - floor(stw));
     for (int v = vb; v < ve; ++v) {
       const T* wp = a.wpart + ((size_t)we * V + v) * 8;
       T rhs[3] = {wp[0], wp[1], wp[2]};
@@ -495,14 +554,23 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
       }
       T* fo = a.fnxt + (size_t)v * a.ldf + 3 * (size_t)e;
       const T fin = a.vtab[2 * V + v];
+      const float ph = sph + (float)a.vtab[3 * V + v];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const T d = mdt * rhs[k] + cold[k];
         res = fmax(res, (double)fabs(d));
-        T fn = f[k] + d;
+        T dk = d;
+        if constexpr (sizeof(T) == 4) {
+          if (a.dither) dk = dither_inc(f[k], dk, __builtin_amdgcn_fractf(wph[k] + ph));
+        }
+        T fn = f[k] + dk;
         fn = fn < T(0) ? T(0) : fn;
         if ((mask >> k) & 1) fn = fin;
-        chg = fmax(chg, (double)fabs(fn - f[k]));
+        T cg = fabs(fn - f[k]);
+        if constexpr (sizeof(T) == 4) {  // the increment, as in k_update
+          if (a.dither && fn != T(0) && !((mask >> k) & 1)) cg = fabs(d);
+        }
+        chg = fmax(chg, (double)cg);
         fo[k] = fn;
         if (use_lds) s_ring[3 * (v - vb) + k] = fn;
       }
@@ -1001,6 +1069,7 @@ struct Ctx {
   int slots = 256;   // update kernel: tile slots (threads per block)
   int depth = 1;     // update kernel: ordinates of f in flight (1 or 2)
   int fuse_mom = 1;  // update kernel also does the first moment pass
+  int dither = 1;    // FP32 bulk: unbiased (dithered) rounding of the f store
   int mom_split = 1; // k_moments wavefronts per node (1, 2 or 4)
   int ntiles = 0, vchunk = 1, nvchunks = 1, nblk_upd = 0, nblk_wall = 0, ngroups = 1,
       nblk_mom = 0, neta = 0, nforce = 0;
@@ -1165,6 +1234,7 @@ int step_once(Ctx& c) {
   u.scal = c.scal; u.res_part = c.res_part; u.res_stride = c.nblk_upd + c.nblk_wall;
   u.E = p.E; u.V = p.V; u.ntiles = c.ntiles; u.vchunk = c.vchunk;
   u.ldf = (long long)c.ldf;
+  u.dither = c.dither;
   tbegin(c, 0);
   launch_update<T>(c, u);
   HIPCK(hipGetLastError());
@@ -1179,6 +1249,7 @@ int step_once(Ctx& c) {
     w.res_part = c.res_part + c.nblk_upd; w.res_stride = c.nblk_upd + c.nblk_wall;
     w.alpha = (T)p.prm.alpha; w.E = p.E; w.V = p.V;
     w.ldf = (long long)c.ldf;
+    w.dither = c.dither;
     w.T_ = p.T; w.ngroups = c.ngroups; w.gsize = c.gsize; w.nwe = (int)p.wall_elems.size();
     w.jacobi = c.gs ? 0 : 1;
     w.diffuse = c.wall_diffuse;
@@ -1305,12 +1376,15 @@ int upload_tables(Ctx& c) {
   if ((st = upload((T**)&c.uny, uny, c.owned))) return st;
   if ((st = upload((T**)&c.ual, ual, c.owned))) return st;
   if ((st = upload(&c.uinfo, uinfo, c.owned))) return st;
-  std::vector<T> vtab(3 * (size_t)V);
+  std::vector<T> vtab(4 * (size_t)V);
   std::vector<double> vtd(7 * (size_t)V);
   for (int v = 0; v < V; ++v) {
     vtab[v] = (T)p.cxa[v];
     vtab[V + v] = (T)p.cya[v];
     vtab[2 * V + v] = (T)p.finf[v];
+    // the FP32 store dither's ordinate phase: a Weyl sequence over v
+    const double ow = 0.7548776662466927 * v;
+    vtab[3 * V + v] = (T)(ow - std::floor(ow));
     vtd[v] = p.cxa[v];
     vtd[V + v] = p.cya[v];
     vtd[2 * V + v] = p.cxb[v];
@@ -1532,6 +1606,7 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
     if (d >= 1 && d <= 2) c.depth = d;
   }
   if (const char* fm = std::getenv("DGBGK_FUSE_MOMENTS")) c.fuse_mom = fm[0] != '0';
+  if (const char* fd = std::getenv("DGBGK_FP32_DITHER")) c.dither = fd[0] != '0';
   const Prep& p = c.p;
   auto cleanup = [&](int code) {
     for (void* q : c.owned) hipFree(q);

@@ -11,7 +11,8 @@
 //   uinfo/unx/uny/ual    the same edges for the update kernel: outward
 //                        normal, neighbour slot and nodes (see kUpd*)
 //   np       T[5][3E]   per node: n*beta2/pi, beta2, u, v, mu (GTEQNF/COMPMU)
-//   vtab     T[3][V]    ordinate velocity (ADVNCE form) and inflow value
+//   vtab     T[4][V]    ordinate velocity (ADVNCE form), inflow value, and
+//                       the FP32 store dither's ordinate phase
 //   mom      double[5][3E] DISND, DISUX, DISUY, DISPS, MU
 #pragma once
 
@@ -63,6 +64,7 @@ struct UpdArgs {
   int res_stride;
   int E, V, ntiles, vchunk;
   long long ldf;  // row stride of f
+  int dither;     // FP32: dithered (unbiased) rounding of the f store
 };
 
 template <typename T>
@@ -92,6 +94,7 @@ struct WallArgs {
   int diffuse;   // alpha = 1: the reflected part has weight 0 and is not read
   int lds_ring;  // one thread per speed ring, in-place reads staged in LDS
   long long ldf;  // row stride of f
+  int dither;     // FP32: dithered (unbiased) rounding of the f store
 };
 
 template <typename T>

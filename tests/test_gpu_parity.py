@@ -215,7 +215,10 @@ def test_change_residual_is_max_step_difference(precision):
     """The record's `change` is max |f^{k+1} - f^k| over every node and
     ordinate after the clamp and the inflow reset: recomputed here from f
     read back before and after each step (wall and inflow elements
-    included)."""
+    included).  In the FP32-bulk form the store rounds with a dither of
+    +-1/2 ulp, and the record holds the step's increment before that
+    rounding, so it is within 2 ulp of the largest f of the read-back
+    difference."""
     from dgbgk.solver import Solver
     case = cylinder_small(elems=800)
     print(f"case change residual precision={precision}", flush=True)
@@ -225,8 +228,8 @@ def test_change_residual_is_max_step_difference(precision):
         r = s.step(1)[0]
         f1 = s.get_f()
         want = np.max(np.abs(f1 - f0))
-        # FP64: the same subtraction; FP32: rounded once more in float
-        tol = 0.0 if precision == 0 else 2 ** -23 * want
+        # FP64: the same subtraction
+        tol = 0.0 if precision == 0 else 2.0 ** -22 * np.max(np.abs(f1))
         assert abs(r["change"] - want) <= tol
         # the reference residual counts the increment before the reset
         assert r["sumres"] >= r["change"] * (1 - 1e-6)
