@@ -115,11 +115,13 @@ def untarget(y: np.ndarray, s: Sample):
 
 def run_to_steady(case, tol: float, max_steps: int, precision=_lib.DGBGK_FP64, seed=None,
                   norm: float | None = None, chunk: int = 1000, log=None,
-                  log_every: int = 20000):
+                  log_every: int = 20000, ref_tol: float | None = None):
     """Steps until change / norm < tol (norm: this run's step-1 change unless
     given).  Wall time covers the steps up to convergence, scaled from the
     timed chunks.  Returns steps, wall seconds, the change history, point
-    fields and forces."""
+    fields and forces.  With ref_tol the run goes on past convergence to
+    change / norm < ref_tol and also returns that state's drag (ref_drag,
+    ref_steps); the fields and forces stay those at tol."""
     from .solver import Solver
     s = Solver(case, precision=precision)
     if seed is not None:
@@ -127,7 +129,9 @@ def run_to_steady(case, tol: float, max_steps: int, precision=_lib.DGBGK_FP64, s
     s.sync()
     hist = []
     first, conv, steps, wall, bad = None, None, 0, 0.0, False
-    while steps < max_steps and conv is None:
+    at_tol = None  # (fields, lift, drag) when change first fell below tol
+    ref_conv = None
+    while steps < max_steps and (conv is None or (ref_tol is not None and ref_conv is None)):
         n = min(chunk, max_steps - steps)
         t0 = time.perf_counter()
         recs = s.step(n)
@@ -142,15 +146,29 @@ def run_to_steady(case, tol: float, max_steps: int, precision=_lib.DGBGK_FP64, s
             hist.append(r["change"])
             if conv is None and r["change"] < tol * (norm if norm is not None else first):
                 conv = steps + i + 1
+            if (ref_tol is not None and ref_conv is None and
+                    r["change"] < ref_tol * (norm if norm is not None else first)):
+                ref_conv = steps + i + 1
         steps += n
         if bad:
             conv = None
             break
+        if conv is not None and at_tol is None:
+            at_tol = (s.point_fields(), *s.forces())
+            wall_at_tol = wall
         if log is not None and steps % log_every == 0:
             log(f"    ... {steps} steps, change {hist[-1]:.3e} "
                 f"(target {tol * (norm if norm is not None else first):.3e})")
-    fields = s.point_fields()
-    lift, drag = s.forces()
+    ref_drag = s.forces()[1] if ref_tol is not None else None
+    if at_tol is not None:
+        fields, lift, drag = at_tol
+        wall = wall_at_tol
+        run_steps = steps
+        steps = min(steps, (conv + chunk - 1) // chunk * chunk)
+    else:
+        fields = s.point_fields()
+        lift, drag = s.forces()
+        run_steps = steps
     s.close()
     # a state with vacuum points (f clamped to zero over a region) has not
     # reached the flow's steady state, however still it has become
@@ -158,9 +176,9 @@ def run_to_steady(case, tol: float, max_steps: int, precision=_lib.DGBGK_FP64, s
         conv, bad = None, True
     return dict(steps=conv if conv is not None else steps, converged=conv is not None,
                 diverged=bad,
-                wall_s=wall * (conv / steps if conv else 1.0), steps_run=steps,
+                wall_s=wall * (conv / steps if conv else 1.0), steps_run=run_steps,
                 first_change=first, change=np.asarray(hist), fields=fields, lift=lift,
-                drag=drag)
+                drag=drag, ref_drag=ref_drag, ref_steps=ref_conv)
 
 
 def _load(path):
@@ -254,10 +272,12 @@ RUNS = {"warm64": (_lib.DGBGK_FP64, True), "cold32": (_lib.DGBGK_FP32, False),
 
 
 def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print,
-             runs=("warm64", "cold32", "warm32")):
+             runs=("warm64", "cold32", "warm32"), ref_tol=None):
     """Per held-out case: the FP64 run from the freestream (cold64) and the
     requested others (FP64/FP32-bulk, from the freestream or from the
-    network's state), all judged by cold64's step-1 change."""
+    network's state), all judged by cold64's step-1 change.  With ref_tol,
+    cold64 goes on to change < ref_tol x its step-1 change, and every run's
+    drag is also compared with that converged drag (drag_rel_ref)."""
     m = load_model(params)
     mesh = mesh_for(elems)
     wd = meshgen.wall_distance(mesh)
@@ -265,7 +285,7 @@ def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print,
     for s in draw(n, seed):
         case = cylinder_case(mesh, s, rorder, torder)
         pred = untarget(m.forward(features(mesh, wd, s)), s)
-        cold64 = run_to_steady(case, tol, max_steps, _lib.DGBGK_FP64, log=log)
+        cold64 = run_to_steady(case, tol, max_steps, _lib.DGBGK_FP64, log=log, ref_tol=ref_tol)
         norm = cold64["first_change"]
         done = dict(cold64=cold64)
         for k in runs:
@@ -274,6 +294,10 @@ def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print,
                                     log=log)
         ref = cold64["fields"]
         rec = dict(mach=s.mach, kn=s.kn, norm=norm)
+        dref = cold64["ref_drag"]
+        if dref is not None:
+            rec["ref_drag"] = float(dref)
+            rec["ref_steps"] = cold64["ref_steps"]
         for k, r in done.items():
             rec[k] = dict(steps=int(r["steps"]), converged=bool(r["converged"]),
                           wall_s=float(r["wall_s"]), drag=float(r["drag"]),
@@ -281,6 +305,8 @@ def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print,
                                          max(abs(cold64["drag"]), 1e-30)),
                           rho_rel_l2=float(np.linalg.norm(r["fields"]["rho"] - ref["rho"]) /
                                            np.linalg.norm(ref["rho"])))
+            if dref is not None:
+                rec[k]["drag_rel_ref"] = float(abs(r["drag"] - dref) / max(abs(dref), 1e-30))
         # how close the network's state is to the steady one
         rec["pred_rho_rel_l2"] = float(np.linalg.norm(pred[0] - ref["rho"]) /
                                        np.linalg.norm(ref["rho"]))
@@ -290,6 +316,8 @@ def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print,
             f"{'' if rec[k]['converged'] else ' (not converged)'}"
             for k in done) + "; drag rel " + ", ".join(
             f"{k} {rec[k]['drag_rel']:.1e}" for k in runs) +
+            ("" if dref is None else "; vs converged FP64 drag " + ", ".join(
+                f"{k} {rec[k]['drag_rel_ref']:.1e}" for k in done)) +
             f"; prediction rho rel L2 {rec['pred_rho_rel_l2']:.1e}")
     return out
 
@@ -306,6 +334,10 @@ def summarise(res):
         out[f"speedup_wall_{k}_vs_cold64"] = med("wall_s", "cold64", k)
         out[f"median_drag_rel_{k}"] = float(np.median([r[k]["drag_rel"] for r in res]))
         out[f"max_drag_rel_{k}"] = float(max(r[k]["drag_rel"] for r in res))
+    if "drag_rel_ref" in res[0]["cold64"]:
+        for k in kinds:
+            out[f"median_drag_rel_ref_{k}"] = float(np.median([r[k]["drag_rel_ref"] for r in res]))
+            out[f"max_drag_rel_ref_{k}"] = float(max(r[k]["drag_rel_ref"] for r in res))
     out["median_pred_rho_rel_l2"] = float(np.median([r["pred_rho_rel_l2"] for r in res]))
     return out
 
@@ -320,6 +352,8 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--ref-tol", type=float, default=0.0,
+                    help="evaluate: also run cold64 on to this tolerance and compare drags with it")
     ap.add_argument("--max-steps", type=int, default=200000)
     ap.add_argument("--time-budget", type=float, default=0.0)
     ap.add_argument("--train-steps", type=int, default=3000)
@@ -346,7 +380,8 @@ def main(argv=None):
                             loss_last=float(losses[-50:].mean()))))
     else:
         res = evaluate(a.elems, a.params, a.n, a.seed, a.rorder, a.torder, a.tol,
-                       a.max_steps, log, runs=tuple(x for x in a.runs.split(",") if x))
+                       a.max_steps, log, runs=tuple(x for x in a.runs.split(",") if x),
+                       ref_tol=a.ref_tol or None)
         summ = summarise(res)
         with open(a.out, "w") as fh:
             json.dump(dict(elems=a.elems, ordinates=a.rorder * a.torder, tol=a.tol,
