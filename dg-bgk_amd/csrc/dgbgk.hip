@@ -64,6 +64,18 @@ __device__ __forceinline__ float dither_inc(float f, float d, float r) {
   return fmaf(fmaf(r, 0x1p-23f, -0x1p-24f), u, d);
 }
 
+// The update pass's f store, non-temporal: the written rows take no L2
+// lines, so the lines a neighbouring tile still has to read as its halo
+// stay (the update's counted reads fall from 1.38x to 1.27x of its
+// algorithmic bytes in FP32, 1.44x to 1.34x in FP64), and the moments
+// pass that follows starts without dirty lines to write back (15% faster
+// in FP32, 10% in FP64).  The wall pass's scattered stores stay plain
+// (non-temporal ones made it slower).
+template <typename T>
+__device__ __forceinline__ void fstore_nt(T* p, T x) {
+  __builtin_nontemporal_store(x, p);
+}
+
 __device__ __forceinline__ float dexp(float x) { return __expf(x); }
 __device__ __forceinline__ double dexp(double x) { return exp(x); }
 
@@ -278,7 +290,7 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
   chg = fmax(chg, fmax(fmax(cg[0], cg[1]), cg[2]));
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    fo[k] = fn[k];
+    fstore_nt(fo + k, fn[k]);
     if (kMom) {
       // MACROS' first pass (src/macros.f90:93-108) on f^{k+1}, this chunk's
       // ordinates, in FP64 (wcx = w cx, wcy = w cy; k_moments sums the same
