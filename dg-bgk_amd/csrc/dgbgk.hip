@@ -39,7 +39,7 @@ __device__ __forceinline__ int xcd_logical(int bid, int nblocks) {
   return xcd * q + min(xcd, r) + loc;
 }
 
-// A node's fixed dither phase in [0, 1): an integer hash of its index
+// An element's fixed dither phase in [0, 1): an integer hash of its index
 // (lowbias32 mixing), the same on every device and every run.
 __device__ __forceinline__ float node_phase(unsigned x) {
   x ^= x >> 16;
@@ -56,13 +56,12 @@ __device__ __forceinline__ float node_phase(unsigned x) {
 // about an ulp per step a plain rounded store drops the increment, and a
 // run near steady state stalls at a state biased by ~ulp / (decay rate per
 // step).  The dither fraction r in [0, 1) runs through a Weyl sequence per
-// node (node, ordinate and step phases summed mod 1), equidistributed over
-// steps.  f = 0 and denormals have no exponent bits and get no dither, so
+// element (element, ordinate and step phases summed mod 1), equidistributed
+// over steps.  f = 0 and denormals have no exponent bits and get no dither, so
 // clamped nodes stay exactly zero.
-__device__ __forceinline__ float dither_inc(float f, float d, float r) {
-  const float u = __uint_as_float(__float_as_uint(f) & 0x7f800000u);  // 2^exponent(f)
-  return fmaf(fmaf(r, 0x1p-23f, -0x1p-24f), u, d);
-}
+// The dither is (r - 1/2) 2^-23 times 2^exponent(f), i.e. (r - 1/2) ulp(f):
+// t = fma(r, 2^-23, -2^-24) once per element and ordinate, then per node
+// d + t * (f with its mantissa bits cleared).
 
 // The update pass's f store, non-temporal: the written rows take no L2
 // lines, so the lines a neighbouring tile still has to read as its halo
@@ -167,7 +166,7 @@ struct SlotState {
   T gxa[3], gya[3], mdt, nx[3], ny[3], al[3];          // phase B (owned slots)
   int info[3], mask, wsl, e;
   int nb1[3], nb2[3];  // LDS index (4 slot + node) of the neighbour's values at k, k1
-  float dph[3];        // FP32 store dither: the nodes' fixed phases
+  float dph;           // FP32 store dither: the element's fixed phase
   bool owned, active;
 };
 
@@ -191,6 +190,7 @@ __device__ __forceinline__ void store4<double>(double* p, double a, double b, do
 template <typename T, bool kMom>
 __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotState<T>& s,
                                                 int v, T ux, T uy, T fin, float ph,
+                                                float dsc0, float dsc1,
                                                 const T (&f)[3],
                                                 T* __restrict__ sf, T dte, T& res, T& chg,
                                                 double wv, double wcx, double wcy,
@@ -257,36 +257,37 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
   // GETINC + ADTHEM + INFLOW (src/getinc.f90:30-42, src/adthem.f90:109-118,
   // src/inflow.f90:165-181); mdt = 6/RJ * DTE
   T* fo = a.fnxt + (size_t)v * a.ldf + 3 * (size_t)s.e;
-  T d[3], fn[3];
+  T d[3], fn[3], cg[3];
+  // FP32: the dithered store with one dither fraction for the element's
+  // three nodes (a shared fraction leaves each node's rounding unbiased);
+  // dsc = (2^-23, -2^-24), or (0, 0) without the dither, which then adds
+  // exactly 0.  The change is the step's own increment, not what the
+  // rounding adds or drops: |d| where the store was not clamped (clamped:
+  // f), and the reset below sets inflow nodes' |f_inf - f|.
+  float t = 0.f;
+  if constexpr (sizeof(T) == 4) t = fmaf(__builtin_amdgcn_fractf(s.dph + ph), dsc0, dsc1);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     d[k] = s.mdt * rhs[k] + cold[k];
     T dk = d[k];
-    if constexpr (sizeof(T) == 4) {
-      if (a.dither) dk = dither_inc(f[k], dk, __builtin_amdgcn_fractf(s.dph[k] + ph));
-    }
+    if constexpr (sizeof(T) == 4)
+      dk = fmaf(t, __uint_as_float(__float_as_uint(f[k]) & 0x7f800000u), dk);
     fn[k] = f[k] + dk;
     fn[k] = fn[k] < T(0) ? T(0) : fn[k];  // NaN stays NaN, as in the reference
+    if constexpr (sizeof(T) == 4)
+      cg[k] = fn[k] != T(0) ? T(fabs(d[k])) : f[k];
+    else
+      cg[k] = fabs(fn[k] - f[k]);
   }
   if (__ballot(s.mask != 0)) {  // inflow nodes: rare; a wave-uniform branch
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      if ((s.mask >> k) & 1) fn[k] = fin;
+      if ((s.mask >> k) & 1) {
+        fn[k] = fin;
+        cg[k] = fabs(fin - f[k]);
+      }
   }
   res = fmax(res, fmax(fmax(fabs(d[0]), fabs(d[1])), fabs(d[2])));
-  T cg[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) cg[k] = fabs(fn[k] - f[k]);
-  if constexpr (sizeof(T) == 4) {
-    // with the dithered store the change is the step's own increment, not
-    // the +-1 ulp its rounding adds: |d| where the store was neither
-    // clamped nor reset
-    if (a.dither) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if (fn[k] != T(0) && !((s.mask >> k) & 1)) cg[k] = fabs(d[k]);
-    }
-  }
   chg = fmax(chg, fmax(fmax(cg[0], cg[1]), cg[2]));
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -348,9 +349,10 @@ k_update(UpdArgs<T> a) {
     s.info[k] = kUpdOwn;
     s.nb1[k] = 4 * slot + k;
     s.nb2[k] = 4 * slot + (k + 1) % 3;
-    s.dph[k] = 0.f;
+
   }
   s.mdt = T(0);
+  s.dph = 0.f;
   s.mask = 0;
   s.wsl = -1;
   if (s.active) {
@@ -385,8 +387,7 @@ k_update(UpdArgs<T> a) {
     }
     s.mask = a.emask[e];
     s.wsl = a.ewslot[e];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) s.dph[k] = node_phase(3u * (unsigned)e + (unsigned)k);
+    s.dph = node_phase((unsigned)e);
   }
   T res = 0, chg = 0;
 
@@ -397,6 +398,7 @@ k_update(UpdArgs<T> a) {
   double qw[kQ], qcx[kQ], qcy[kQ];
   const double stw = a.scal[6] * 0.6180339887498949;  // Weyl step phase
   const float sph = (float)(stw - floor(stw));
+  const float dsc0 = a.dither ? 0x1p-23f : 0.f, dsc1 = a.dither ? -0x1p-24f : 0.f;
   const T* fe = a.fcur + 3 * (size_t)e;
 #pragma unroll
   for (int j = 0; j < kQ; ++j) {
@@ -445,7 +447,7 @@ k_update(UpdArgs<T> a) {
             qcy[jn] = ldc(a.vtd, 6 * V + vn);
           }
         }
-        update_ordinate<T, kMom>(a, s, v, qx[j], qy[j], qi[j], qp[j], q[j], s_f[j], dte,
+        update_ordinate<T, kMom>(a, s, v, qx[j], qy[j], qi[j], qp[j], dsc0, dsc1, q[j], s_f[j], dte,
                                  res, chg, qw[j], qcx[j], qcy[j], m0, m1, m2);
       }
     }
@@ -509,9 +511,7 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
     const int mask = a.emask[e];
     const T alpha = a.alpha;
     // FP32 store dither (as in k_update): node, ordinate and step phases
-    float wph[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) wph[k] = node_phase(3u * (unsigned)e + (unsigned)k);
+    const float wph = node_phase((unsigned)e);
     const double stw = a.scal[6] * 0.6180339887498949;
     const float sph = (float)(stw - floor(stw));
     for (int v = vb; v < ve; ++v) {
@@ -567,20 +567,22 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
       T* fo = a.fnxt + (size_t)v * a.ldf + 3 * (size_t)e;
       const T fin = a.vtab[2 * V + v];
       const float ph = sph + (float)a.vtab[3 * V + v];
+      const float wt = fmaf(__builtin_amdgcn_fractf(wph + ph), a.dither ? 0x1p-23f : 0.f,
+                            a.dither ? -0x1p-24f : 0.f);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const T d = mdt * rhs[k] + cold[k];
         res = fmax(res, (double)fabs(d));
         T dk = d;
         if constexpr (sizeof(T) == 4) {
-          if (a.dither) dk = dither_inc(f[k], dk, __builtin_amdgcn_fractf(wph[k] + ph));
+          dk = fmaf(wt, __uint_as_float(__float_as_uint(f[k]) & 0x7f800000u), dk);
         }
         T fn = f[k] + dk;
         fn = fn < T(0) ? T(0) : fn;
         if ((mask >> k) & 1) fn = fin;
         T cg = fabs(fn - f[k]);
         if constexpr (sizeof(T) == 4) {  // the increment, as in k_update
-          if (a.dither && fn != T(0) && !((mask >> k) & 1)) cg = fabs(d);
+          if (fn != T(0) && !((mask >> k) & 1)) cg = fabs(d);
         }
         chg = fmax(chg, (double)cg);
         fo[k] = fn;
