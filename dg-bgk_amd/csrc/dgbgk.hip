@@ -163,7 +163,7 @@ __device__ __forceinline__ void nsum(double& s, double& c, double x) {
 template <typename T>
 struct SlotState {
   T gx[3], gy[3], pA[3], pB[3], pU[3], pV[3], pMd[3];  // phase A (all slots)
-  T gxa[3], gya[3], mdt, nx[3], ny[3], al[3];          // phase B (owned slots)
+  T gxa[3], gya[3], mdt, nx[3], ny[3];                 // phase B (owned slots)
   int info[3], mask, wsl, e;
   int nb1[3], nb2[3];  // LDS index (4 slot + node) of the neighbour's values at k, k1
   float dph;           // FP32 store dither: the element's fixed phase
@@ -238,13 +238,13 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
   for (int k = 0; k < 3; ++k) {
     const int k1 = (k + 1) % 3;
     const int in = s.info[k];
-    const T cn = ux * s.nx[k] + uy * s.ny[k];
-    const bool own = (in & kUpdOwn) || cn > T(0) || ((in & kUpdRight) && cn == T(0));
+    // w = -(len/6) c.n from the edge's (len/6) n: c.n > 0 exactly when w < 0
+    const T w = -(ux * s.nx[k] + uy * s.ny[k]);
+    const bool own = (in & kUpdOwn) || w < T(0) || ((in & kUpdRight) && w == T(0));
     // the neighbour's values are read whichever side is upwind (fixed LDS
     // indices, no address arithmetic); the element's own are in registers
     const T n1 = sf[s.nb1[k]], n2 = sf[s.nb2[k]];
     const T g1 = own ? h[k] : n1, g2 = own ? h[k1] : n2;
-    const T w = -(s.al[k] * cn);
     rhs[k] += w * (T(2) * g1 + g2);
     rhs[k1] += w * (g1 + T(2) * g2);
   }
@@ -273,19 +273,21 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
     if constexpr (sizeof(T) == 4)
       dk = fmaf(t, __uint_as_float(__float_as_uint(f[k]) & 0x7f800000u), dk);
     fn[k] = f[k] + dk;
-    fn[k] = fn[k] < T(0) ? T(0) : fn[k];  // NaN stays NaN, as in the reference
-    if constexpr (sizeof(T) == 4)
-      cg[k] = fn[k] != T(0) ? T(fabs(d[k])) : f[k];
-    else
-      cg[k] = fabs(fn[k] - f[k]);
+    const bool neg = fn[k] < T(0);
+    fn[k] = neg ? T(0) : fn[k];  // NaN stays NaN, as in the reference
+    if constexpr (sizeof(T) == 4) cg[k] = neg ? f[k] : T(fabs(d[k]));
   }
   if (__ballot(s.mask != 0)) {  // inflow nodes: rare; a wave-uniform branch
 #pragma unroll
     for (int k = 0; k < 3; ++k)
       if ((s.mask >> k) & 1) {
         fn[k] = fin;
-        cg[k] = fabs(fin - f[k]);
+        if constexpr (sizeof(T) == 4) cg[k] = fabs(fin - f[k]);
       }
+  }
+  if constexpr (sizeof(T) == 8) {  // FP64: the change as stored
+#pragma unroll
+    for (int k = 0; k < 3; ++k) cg[k] = fabs(fn[k] - f[k]);
   }
   res = fmax(res, fmax(fmax(fabs(d[0]), fabs(d[1])), fabs(d[2])));
   chg = fmax(chg, fmax(fmax(cg[0], cg[1]), cg[2]));
@@ -345,7 +347,7 @@ k_update(UpdArgs<T> a) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     s.gx[k] = s.gy[k] = s.pA[k] = s.pB[k] = s.pU[k] = s.pV[k] = s.pMd[k] = T(0);
-    s.gxa[k] = s.gya[k] = s.nx[k] = s.ny[k] = s.al[k] = T(0);
+    s.gxa[k] = s.gya[k] = s.nx[k] = s.ny[k] = T(0);
     s.info[k] = kUpdOwn;
     s.nb1[k] = 4 * slot + k;
     s.nb2[k] = 4 * slot + (k + 1) % 3;
@@ -383,7 +385,6 @@ k_update(UpdArgs<T> a) {
       }
       s.nx[k] = a.unx[k * E + e];
       s.ny[k] = a.uny[k * E + e];
-      s.al[k] = a.ual[k * E + e];
     }
     s.mask = a.emask[e];
     s.wsl = a.ewslot[e];
@@ -1066,7 +1067,7 @@ struct Ctx {
   void* f[2] = {nullptr, nullptr};
   int cur = 0;
   void *geo = nullptr, *enx = nullptr, *eny = nullptr, *ealen = nullptr, *np = nullptr,
-       *unx = nullptr, *uny = nullptr, *ual = nullptr,
+       *unx = nullptr, *uny = nullptr,
        *vtab = nullptr, *refl_t = nullptr, *wpart = nullptr;
   int *einfo = nullptr, *eslot = nullptr, *emask = nullptr, *ewslot = nullptr, *uinfo = nullptr,
       *refl_i = nullptr, *wall_bside = nullptr, *tiles = nullptr, *halo = nullptr,
@@ -1242,7 +1243,7 @@ int step_once(Ctx& c) {
   }
   UpdArgs<T> u;
   u.fcur = fc; u.fnxt = fn; u.geo = (const T*)c.geo; u.uinfo = c.uinfo;
-  u.unx = (const T*)c.unx; u.uny = (const T*)c.uny; u.ual = (const T*)c.ual;
+  u.unx = (const T*)c.unx; u.uny = (const T*)c.uny;
   u.emask = c.emask; u.ewslot = c.ewslot; u.np = (const T*)c.np; u.vtab = (const T*)c.vtab;
   u.tiles = c.tiles; u.halo = c.halo; u.wpart = (T*)c.wpart; u.vtd = c.vtd; u.msum = c.msum;
   u.scal = c.scal; u.res_part = c.res_part; u.res_stride = c.nblk_upd + c.nblk_wall;
@@ -1341,7 +1342,7 @@ int upload_tables(Ctx& c) {
   const size_t N3 = 3 * (size_t)E;
   std::vector<T> geo(8 * (size_t)E), enx(3 * (size_t)E), eny(3 * (size_t)E), eal(3 * (size_t)E);
   std::vector<int> einfo(3 * (size_t)E), eslot(3 * (size_t)E), emask(E), ewslot(E);
-  std::vector<T> unx(3 * (size_t)E), uny(3 * (size_t)E), ual(3 * (size_t)E);
+  std::vector<T> unx(3 * (size_t)E), uny(3 * (size_t)E);
   std::vector<int> uinfo(3 * (size_t)E);
   for (int e = 0; e < E; ++e) {
     const double* g = &p.geome[7 * (size_t)e];
@@ -1358,20 +1359,20 @@ int upload_tables(Ctx& c) {
       enx[i] = (T)ed.nx;
       eny[i] = (T)ed.ny;
       eal[i] = (T)ed.alen;
-      // update kernel: outward normal (the right element's is the side's
-      // normal negated), the neighbour's nodes at this edge's nodes k and
-      // k+1, and no flux through inflow and wall sides
-      const T sg = ed.sign > 0 ? T(1) : T(-1);
-      unx[i] = sg * (T)ed.nx;
-      uny[i] = sg * (T)ed.ny;
+      // update kernel: (len/6) times the outward normal (the right
+      // element's is the side's normal negated), rounded once; zero on
+      // inflow and wall sides (no flux here).  The neighbour's nodes at
+      // this edge's nodes k and k+1.
+      const double sg = ed.sign > 0 ? 1.0 : -1.0;
+      const bool flux = ed.kind == kInterior || ed.kind == kOutflow;
+      unx[i] = flux ? (T)(sg * ed.alen * ed.nx) : T(0);
+      uny[i] = flux ? (T)(sg * ed.alen * ed.ny) : T(0);
       if (ed.kind == kInterior) {
         const int nbk = ed.sign > 0 ? ed.b1 : ed.b2, nbk1 = ed.sign > 0 ? ed.b2 : ed.b1;
         uinfo[i] = (nbk << kUpdNbK) | (nbk1 << kUpdNbK1) | (ed.sign > 0 ? 0 : kUpdRight) |
                    (p.nbr_slot[3 * (size_t)e + k] << kUpdSlot);
-        ual[i] = (T)ed.alen;
       } else {
         uinfo[i] = kUpdOwn;
-        ual[i] = ed.kind == kOutflow ? (T)ed.alen : T(0);
       }
     }
     emask[e] = p.inflow_mask[e];
@@ -1388,7 +1389,6 @@ int upload_tables(Ctx& c) {
   if ((st = upload(&c.ewslot, ewslot, c.owned))) return st;
   if ((st = upload((T**)&c.unx, unx, c.owned))) return st;
   if ((st = upload((T**)&c.uny, uny, c.owned))) return st;
-  if ((st = upload((T**)&c.ual, ual, c.owned))) return st;
   if ((st = upload(&c.uinfo, uinfo, c.owned))) return st;
   std::vector<T> vtab(4 * (size_t)V);
   std::vector<double> vtd(7 * (size_t)V);

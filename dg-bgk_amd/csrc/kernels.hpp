@@ -8,8 +8,9 @@
 //   einfo    int[3][E]  per element edge: kind | a1 | a2 | b1 | b2 | left
 //   eslot    int[3][E]  interior: neighbour's LDS slot; wall: wall-side slot
 //   enx/eny/ealen T[3][E] side normal and length/6 of each element edge
-//   uinfo/unx/uny/ual    the same edges for the update kernel: outward
-//                        normal, neighbour slot and nodes (see kUpd*)
+//   uinfo/unx/uny        the same edges for the update kernel: (len/6)
+//                        times the outward normal (zero on inflow and wall
+//                        sides), neighbour slot and nodes (see kUpd*)
 //   np       T[5][3E]   per node: n*beta2/pi, beta2, u, v, mu (GTEQNF/COMPMU)
 //   vtab     T[4][V]    ordinate velocity (ADVNCE form), inflow value, and
 //                       the FP32 store dither's ordinate phase
@@ -47,9 +48,8 @@ struct UpdArgs {
   T* __restrict__ fnxt;
   const T* __restrict__ geo;
   const int* __restrict__ uinfo;   // [3][E] edge words
-  const T* __restrict__ unx;       // [3][E] outward unit normal of the edge
-  const T* __restrict__ uny;
-  const T* __restrict__ ual;       // [3][E] length/6; 0 on inflow and wall sides
+  const T* __restrict__ unx;       // [3][E] (len/6) x outward normal; 0 on inflow
+  const T* __restrict__ uny;       //        and wall sides
   const int* __restrict__ emask;   // inflow bits 0..2
   const int* __restrict__ ewslot;  // wall element slot or -1
   const T* __restrict__ np;
