@@ -233,7 +233,7 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
   // flux from the right element; with the right element's outward normal
   // that is the same expression.  Upwind: own values when c.n > 0 (on a
   // tie the right element's values, as in the reference), always on a
-  // boundary side; inflow and wall sides have zero length here.
+  // boundary side; inflow and wall sides carry a zero (len/6) n here.
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int k1 = (k + 1) % 3;

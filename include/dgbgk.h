@@ -72,7 +72,9 @@ typedef struct dgbgk_step_info {
                     the clamp and the inflow reset (no reference counterpart:
                     SUMRES counts the increment before the reset, so at
                     nodes held at the inflow state it never vanishes; this
-                    is the steady-state measure) */
+                    is the steady-state measure).  DGBGK_FP32: the step's
+                    increment before the store's rounding, |d| (f where
+                    clamped, |f_inf - f| where reset) */
 } dgbgk_step_info;
 
 typedef struct dgbgk_ctx dgbgk_ctx;
