@@ -18,6 +18,11 @@ launch time over K more steps launched with HIP events around every kernel
 on the library's stream.
 cpu_baseline: the FP64 C oracle (one core) on a bounded sample of the same
 workload (2 steps).
+time_to_residual (cylinder configuration; --no-ttr skips it): steps and wall
+time to a step change of 1e-6 of the plain run's step-1 change on the same
+geometry held at the freestream on all box sides, FP64 from the freestream
+against FP32-bulk from the committed warm-start network's state (about a
+minute).
 """
 import argparse
 import json
@@ -60,6 +65,46 @@ def copy_calibration(nbytes: int):
     print(f"copied {nbytes} bytes x 3", file=sys.stderr)
 
 
+def time_to_residual():
+    """The metric's second half on the headline geometry: steps and wall time
+    to a step change of 1e-6 of the plain run's step-1 change, FP64 from the
+    freestream (plain) against FP32-bulk from the committed network's state
+    (learned), on the 101,753-triangle cylinder (16 x 32 ordinates, freestream
+    held on all box sides, as steady runs need).  One held-out case near the
+    median of the eight evaluated: Mach 0.82, Kn 0.17."""
+    from dgbgk import _lib, meshgen, warmstart
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+    params = os.path.join(ROOT, "dg-bgk_amd", "dgbgk", "data", "warmstart_cylinder.npz")
+    mesh = warmstart.mesh_for(100_000)
+    smp = warmstart.Sample(0.82, 0.17)
+    case = warmstart.cylinder_case(mesh, smp, 16, 32)
+    log(f"# time to residual: cylinder E={mesh.nelem}, Mach {smp.mach}, Kn {smp.kn}")
+    plain = warmstart.run_to_steady(case, 1e-6, 400_000, _lib.DGBGK_FP64, log=log)
+    log(f"#   plain FP64: {plain['steps']} steps, {plain['wall_s']:.1f} s")
+    t0 = time.perf_counter()
+    m = warmstart.load_model(params)
+    pred = warmstart.untarget(m.forward(warmstart.features(mesh, meshgen.wall_distance(mesh), smp)),
+                              smp)
+    predict_s = time.perf_counter() - t0
+    learned = warmstart.run_to_steady(case, 1e-6, 400_000, _lib.DGBGK_FP32, pred,
+                                      plain["first_change"], log=log)
+    log(f"#   learned FP32-bulk: {learned['steps']} steps, {learned['wall_s']:.1f} s")
+    wl = learned["wall_s"] + predict_s
+    return {"case": f"cylinder {mesh.nelem} triangles x 16x32 ordinates, Mach 0.82, Kn 0.17, "
+                    "freestream on all box sides",
+            "tol": 1e-6,
+            "plain_fp64": {"steps": int(plain["steps"]), "wall_s": plain["wall_s"],
+                           "converged": bool(plain["converged"]), "drag": plain["drag"]},
+            "learned_fp32": {"steps": int(learned["steps"]), "wall_s": wl,
+                             "network_s": predict_s,
+                             "converged": bool(learned["converged"]), "drag": learned["drag"]},
+            "speedup_wall": plain["wall_s"] / wl,
+            "speedup_steps": plain["steps"] / max(learned["steps"], 1),
+            "drag_rel": abs(learned["drag"] - plain["drag"]) / abs(plain["drag"])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +117,9 @@ def main():
     ap.add_argument("--elems", type=int, default=0, help="override the mesh size")
     ap.add_argument("--vchunk", type=int, default=0, help="ordinates per update block (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ttr", action="store_true",
+                    help="skip the time-to-residual comparison (learned FP32-bulk vs plain "
+                         "FP64, about a minute; run with the cylinder configuration only)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run three device-to-device copies of BYTES bytes (the "
@@ -180,6 +228,8 @@ def main():
                      "logres_last": recs[-1]["logres"]},
         "cpu_baseline": cpu,
     }
+    if args.config == "cylinder" and not args.no_ttr and not args.elems:
+        out["time_to_residual"] = time_to_residual()
     print(json.dumps(out))
 
 
