@@ -147,6 +147,26 @@ def test_moment_split(split, prec, monkeypatch):
     assert np.array_equal(s2.get_f(), s.get_f())
 
 
+def test_launch_config_reports_the_choices(monkeypatch):
+    """dgbgk_launch_config: the tiling, prefetch depth, ordinate chunks and
+    moments split the library chose, and the environment overrides."""
+    from dgbgk import _lib
+    from dgbgk.solver import Solver
+    case = cylinder_small()
+    print("case launch configuration", flush=True)
+    for prec, depth in ((_lib.DGBGK_FP64, 1), (_lib.DGBGK_FP32, 2)):
+        c = Solver(case, precision=prec).launch_config()
+        assert c["slots"] == 256 and c["depth"] == depth
+        assert c["ntiles"] >= -(-case.mesh.nelem // 256)
+        assert c["nvchunks"] == -(-case.nvel // c["vchunk"])
+        assert c["mom_split"] == 4  # a small mesh: four wavefronts per node
+    monkeypatch.setenv("DGBGK_MOM_SPLIT", "2")
+    monkeypatch.setenv("DGBGK_PREFETCH", "2")
+    c = Solver(case, precision=_lib.DGBGK_FP64, vchunk=7).launch_config()
+    assert c["mom_split"] == 2 and c["depth"] == 2
+    assert c["vchunk"] == 7 and c["nvchunks"] == -(-case.nvel // 7)
+
+
 def test_fp32_bulk_c1():
     from dgbgk import _lib
     s, o = _pair(c1(), precision=_lib.DGBGK_FP32)
