@@ -184,7 +184,9 @@ __device__ __forceinline__ void store4<double>(double* p, double a, double b, do
 }
 
 // One ordinate v of one slot.  f: the slot's three node values at v; sf:
-// this ordinate's LDS buffer, [slot][4] = f at the three nodes and U.
+// this ordinate's LDS buffer, [slot][4] = g = f - U at the three nodes and
+// a zero.  ph: the ordinate's and step's dither phase, dsc0/dsc1 the
+// dither's scale (zero without it).
 // Contains the block's one barrier per ordinate, so every thread of the
 // block calls it for every v.
 template <typename T, bool kMom>
@@ -197,7 +199,6 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
                                                 double (&m0)[3], double (&m1)[3],
                                                 double (&m2)[3]) {
   const int slot = threadIdx.x;
-  const size_t N3 = 3 * (size_t)a.E;
   const T c13 = T(1) / T(3);
   // collision term times DTE: pMd = MU * DTE (src/getinc.f90:34-41)
   // phase A runs on every slot without a branch: slots without an element
