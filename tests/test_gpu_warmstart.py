@@ -34,7 +34,11 @@ def test_trained_network_shortens_the_run_to_steady_state(tmp_path):
     assert losses[-50:].mean() < 0.1 * losses[:20].mean()
 
     case = warmstart.cylinder_case(mesh, held_out, rorder, torder)
-    cold = warmstart.run_to_steady(case, tol, 60000)
+    cold = warmstart.run_to_steady(case, tol, 60000, ref_tol=1e-8)
+    # the run went on past tol to the reference tolerance; fields and drag
+    # are those at tol, ref_drag the converged one
+    assert cold["ref_steps"] is not None and cold["ref_steps"] > cold["steps"]
+    assert abs(cold["ref_drag"] - cold["drag"]) <= 2e-3 * abs(cold["ref_drag"])
     pred = warmstart.untarget(m.forward(warmstart.features(mesh, wd, held_out)), held_out)
     warm = warmstart.run_to_steady(case, tol, 60000, seed=pred, norm=cold["first_change"])
     print(f"held out: cold {cold['steps']} steps, warm {warm['steps']} steps, "
