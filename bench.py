@@ -115,6 +115,12 @@ def main():
                     help="cylinder: 100k triangles x 512 ordinates (headline); "
                          "particle: 1M triangles x 2048 ordinates")
     ap.add_argument("--elems", type=int, default=0, help="override the mesh size")
+    ap.add_argument("--mesh", default="",
+                    help="a mesh file in the reference's .plt format, run with the "
+                         "cylinder configuration's ordinates and flow conditions")
+    ap.add_argument("--shuffle", type=int, default=-1, metavar="SEED",
+                    help="renumber the mesh's elements at random (an input mesh in "
+                         "arbitrary element order)")
     ap.add_argument("--vchunk", type=int, default=0, help="ordinates per update block (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ttr", action="store_true",
@@ -134,11 +140,17 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from dgbgk import _lib, cases
+    from dgbgk import _lib, cases, io, meshgen
     from dgbgk.solver import Solver
 
     t0 = time.time()
-    if args.config == "cylinder":
+    if args.mesh:
+        case = cases.cylinder_flow(target_elems=2000)
+        case.mesh = io.read_plt(args.mesh)
+        wl = (f"mesh {os.path.basename(args.mesh)} x 16x32 ordinates, Mach 0.5, Kn 0.05 "
+              "(cylinder flow conditions)")
+        mach, kn = 0.5, 0.05
+    elif args.config == "cylinder":
         case = cases.cylinder_flow(target_elems=args.elems or 100_000)
         wl = "cylinder flow, 20Dx10D box, ~100k triangles x 16x32 ordinates, Mach 0.5, Kn 0.05"
         mach, kn = 0.5, 0.05
@@ -146,6 +158,9 @@ def main():
         case = cases.particle_drag(target_elems=args.elems or 1_000_000)
         wl = "particle drag, ~1M triangles x 32x64 ordinates, Mach 0.3, Kn 1"
         mach, kn = 0.3, 1.0
+    if args.shuffle >= 0:
+        case.mesh, _ = meshgen.shuffle_elements(case.mesh, args.shuffle)
+        wl += f", elements in random order (seed {args.shuffle})"
     prec = _lib.DGBGK_FP64 if args.precision == "fp64" else _lib.DGBGK_FP32
     s = Solver(case, precision=prec, vchunk=args.vchunk)
     setup_s = time.time() - t0
@@ -184,7 +199,8 @@ def main():
     avg_upd_s = (upd_ms / max(upd_n, 1)) / 1e3
     alg_bytes = 2.0 * bpv * dofv  # one read + one write of f per update
     achieved = alg_bytes / avg_upd_s / 1e9 if upd_n else None
-    traffic = _traffic_from_profiles(f"{args.config}:{E}x{V}", args.precision)
+    traffic = (_traffic_from_profiles(f"{args.config}:{E}x{V}", args.precision)
+               if not args.mesh and args.shuffle < 0 else None)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -228,7 +244,8 @@ def main():
                      "logres_last": recs[-1]["logres"]},
         "cpu_baseline": cpu,
     }
-    if args.config == "cylinder" and not args.no_ttr and not args.elems:
+    if (args.config == "cylinder" and not args.no_ttr and not args.elems and not args.mesh
+            and args.shuffle < 0):
         out["time_to_residual"] = time_to_residual()
     print(json.dumps(out))
 

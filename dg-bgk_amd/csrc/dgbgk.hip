@@ -996,28 +996,32 @@ __global__ void __launch_bounds__(256) k_seed(T* __restrict__ f, const int* __re
   f[(size_t)v * ldf + k] = (T)(n1 * f0);
 }
 
-// Layout conversions between the reference's DISNF(3,V,E) (double) and
-// the device's velocity-major T[V][3E].
+// Layout conversions between the reference's DISNF(3,V,E) (double, input
+// element numbering) and the device's velocity-major T[V][ldf] (storage
+// order; eorder maps a storage position to its input element, or is null
+// when the two coincide).
 template <typename T>
-__global__ void k_to_device(const double* __restrict__ in, T* __restrict__ out, int E, int V,
-                            long long ldf) {
+__global__ void k_to_device(const double* __restrict__ in, T* __restrict__ out,
+                            const int* __restrict__ eorder, int E, int V, long long ldf) {
   const size_t N3 = 3 * (size_t)E;
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int v = blockIdx.y;
   if (k >= N3) return;
   const size_t e = k / 3, a = k % 3;
-  out[(size_t)v * ldf + k] = (T)in[(e * V + v) * 3 + a];
+  const size_t ei = eorder ? (size_t)eorder[e] : e;
+  out[(size_t)v * ldf + k] = (T)in[(ei * V + v) * 3 + a];
 }
 
 template <typename T>
-__global__ void k_to_host(const T* __restrict__ in, double* __restrict__ out, int E, int V,
-                          long long ldf) {
+__global__ void k_to_host(const T* __restrict__ in, double* __restrict__ out,
+                          const int* __restrict__ eorder, int E, int V, long long ldf) {
   const size_t N3 = 3 * (size_t)E;
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int v = blockIdx.y;
   if (k >= N3) return;
   const size_t e = k / 3, a = k % 3;
-  out[(e * V + v) * 3 + a] = (double)in[(size_t)v * ldf + k];
+  const size_t ei = eorder ? (size_t)eorder[e] : e;
+  out[(ei * V + v) * 3 + a] = (double)in[(size_t)v * ldf + k];
 }
 
 // ---------------------------------------------------------------------------
@@ -1058,7 +1062,18 @@ int alloc(X** d, size_t n, std::vector<void*>& owned) {
 }  // namespace
 
 struct Ctx {
-  Prep p;
+  Prep p;  // built on the mesh in storage order
+  // storage position -> input element (0-based); empty when the input
+  // numbering is kept.  With a renumbering, the input mesh and setup
+  // arguments are kept for the reference tables.
+  std::vector<int> eorder;
+  int* d_eorder = nullptr;
+  struct Input {
+    std::vector<int> intma, bsido;
+    std::vector<double> coord, eta, w;
+    dgbgk_params prm{};
+    int rorder = 0, torder = 0;
+  } in;
   int prec = 0;   // 0 fp64, 1 fp32
   int gs = 1;     // in-place sweep order
   size_t tsz = 8;
@@ -1512,8 +1527,15 @@ struct dgbgk_ctx {
   Ctx c;
 };
 struct dgbgk_setup {
-  Prep p;
+  Prep p;          // input numbering: the reference tables
+  int ntiles = 0;  // update tiles in the device's storage order
 };
+
+// Elements are renumbered along a Hilbert curve unless DGBGK_REORDER=0.
+static bool reorder_enabled() {
+  const char* r = std::getenv("DGBGK_REORDER");
+  return !(r && r[0] == '0');
+}
 
 static Params to_params(const dgbgk_params* prm) {
   Params q;
@@ -1544,6 +1566,20 @@ int dgbgk_setup_create(int nelem, int npoin, int nboun, const int* intma, const 
     delete s;
     return fail(st, msg);
   }
+  s->ntiles = (int)s->p.tile_e0.size();
+  std::vector<int> order;
+  if (reorder_enabled() && hilbert_order(nelem, npoin, intma, coord, order)) {
+    std::vector<int> im, bs;
+    renumber_elements(nelem, nboun, intma, bsido, order, im, bs);
+    Prep q;
+    st = prepare(nelem, npoin, nboun, im.data(), coord, bs.data(), rorder, torder, eta, w,
+                 to_params(prm), kSlots, kSlots, q, msg);
+    if (st) {
+      delete s;
+      return fail(st, msg);
+    }
+    s->ntiles = (int)q.tile_e0.size();
+  }
   *out = s;
   return kOk;
 }
@@ -1553,7 +1589,7 @@ int dgbgk_setup_sizes(dgbgk_setup* s, int* nside, int* nvel, int* ntiles, int* n
   if (!s) return fail(kErrArg, "null setup");
   if (nside) *nside = s->p.S;
   if (nvel) *nvel = s->p.V;
-  if (ntiles) *ntiles = (int)s->p.tile_e0.size();
+  if (ntiles) *ntiles = s->ntiles;
   if (nwall_sides) *nwall_sides = (int)s->p.wall_sides.size();
   if (ring_separable) *ring_separable = s->p.ring_separable ? 1 : 0;
   return kOk;
@@ -1601,11 +1637,32 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   if (const char* ps = std::getenv("DGBGK_SLOTS")) c.slots = std::atoi(ps) == 512 ? 512 : 256;
   int owned_cap = c.slots;
   if (const char* po = std::getenv("DGBGK_TILE_OWNED")) owned_cap = std::atoi(po);
-  int st = prepare(nelem, npoin, nboun, intma, coord, bsido, rorder, torder, eta, w,
+  // the device works on the mesh renumbered into storage order (compact
+  // tiles); the ABI's per-element arrays stay in the input numbering
+  const int* im = intma;
+  const int* bs = bsido;
+  std::vector<int> im_r, bs_r;
+  if (reorder_enabled() && hilbert_order(nelem, npoin, intma, coord, c.eorder)) {
+    renumber_elements(nelem, nboun, intma, bsido, c.eorder, im_r, bs_r);
+    im = im_r.data();
+    bs = bs_r.data();
+  }
+  int st = prepare(nelem, npoin, nboun, im, coord, bs, rorder, torder, eta, w,
                    to_params(prm), c.slots, owned_cap, c.p, msg);
   if (st) {
     delete h;
     return fail(st, msg);
+  }
+  if (!c.eorder.empty()) {
+    csr_input_order(c.p, c.eorder);
+    c.in.intma.assign(intma, intma + 3 * (size_t)nelem);
+    c.in.bsido.assign(bsido, bsido + 6 * (size_t)std::max(nboun, 0));
+    c.in.coord.assign(coord, coord + 2 * (size_t)npoin);
+    c.in.eta.assign(eta, eta + rorder);
+    c.in.w.assign(w, w + rorder);
+    c.in.prm = *prm;
+    c.in.rorder = rorder;
+    c.in.torder = torder;
   }
   c.prec = prm->precision;
   c.gs = prm->order == DGBGK_ORDER_REFERENCE ? 1 : 0;
@@ -1707,6 +1764,7 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   c.nforce = (int)p.force_p1.size();
   st = c.prec ? upload_tables<float>(c) : upload_tables<double>(c);
   if (st) return cleanup(st);
+  if (!c.eorder.empty() && (st = upload(&c.d_eorder, c.eorder, c.owned))) return cleanup(st);
   if ((st = alloc(&c.res_part, 2 * ((size_t)c.nblk_upd + c.nblk_wall), c.owned)))
     return cleanup(st);
   if ((st = alloc(&c.msum, (size_t)c.nvchunks * 3 * 3 * (size_t)p.E, c.owned)))
@@ -1792,10 +1850,11 @@ int dgbgk_get_f(dgbgk_ctx* h, double* f) {
   dim3 g((unsigned)((N3 + 255) / 256), c.p.V);
   if (c.prec)
     hipLaunchKernelGGL(k_to_host<float>, g, dim3(256), 0, c.stream, (const float*)c.f[c.cur], d,
-                       c.p.E, c.p.V, (long long)c.ldf);
+                       (const int*)c.d_eorder, c.p.E, c.p.V, (long long)c.ldf);
   else
     hipLaunchKernelGGL(k_to_host<double>, g, dim3(256), 0, c.stream,
-                       (const double*)c.f[c.cur], d, c.p.E, c.p.V, (long long)c.ldf);
+                       (const double*)c.f[c.cur], d, (const int*)c.d_eorder, c.p.E, c.p.V,
+                       (long long)c.ldf);
   hipError_t e1 = hipMemcpyAsync(f, d, n * sizeof(double), hipMemcpyDeviceToHost, c.stream);
   hipError_t e2 = hipStreamSynchronize(c.stream);
   hipFree(d);
@@ -1814,10 +1873,11 @@ int dgbgk_set_f(dgbgk_ctx* h, const double* f) {
   dim3 g((unsigned)((N3 + 255) / 256), c.p.V);
   if (c.prec)
     hipLaunchKernelGGL(k_to_device<float>, g, dim3(256), 0, c.stream, d, (float*)c.f[c.cur],
-                       c.p.E, c.p.V, (long long)c.ldf);
+                       (const int*)c.d_eorder, c.p.E, c.p.V, (long long)c.ldf);
   else
     hipLaunchKernelGGL(k_to_device<double>, g, dim3(256), 0, c.stream, d,
-                       (double*)c.f[c.cur], c.p.E, c.p.V, (long long)c.ldf);
+                       (double*)c.f[c.cur], (const int*)c.d_eorder, c.p.E, c.p.V,
+                       (long long)c.ldf);
   int st = do_refresh(c);
   hipError_t e2 = hipStreamSynchronize(c.stream);
   hipFree(d);
@@ -1854,11 +1914,17 @@ int dgbgk_node_fields(dgbgk_ctx* h, double* disnd, double* disux, double* disuy,
   Ctx& c = h->c;
   const size_t N3 = 3 * (size_t)c.p.E;
   double* dst[5] = {disnd, disux, disuy, disps, mu};
-  for (int i = 0; i < 5; ++i)
-    if (dst[i])
-      HIPCK(hipMemcpyAsync(dst[i], c.mom + i * N3, N3 * sizeof(double), hipMemcpyDeviceToHost,
-                           c.stream));
-  HIPCK(hipStreamSynchronize(c.stream));
+  std::vector<double> tmp(c.eorder.empty() ? 0 : N3);
+  for (int i = 0; i < 5; ++i) {
+    if (!dst[i]) continue;
+    double* to = c.eorder.empty() ? dst[i] : tmp.data();
+    HIPCK(hipMemcpyAsync(to, c.mom + i * N3, N3 * sizeof(double), hipMemcpyDeviceToHost,
+                         c.stream));
+    HIPCK(hipStreamSynchronize(c.stream));
+    if (!c.eorder.empty())
+      for (size_t e = 0; e < (size_t)c.p.E; ++e)
+        for (int a = 0; a < 3; ++a) dst[i][3 * (size_t)c.eorder[e] + a] = tmp[3 * e + a];
+  }
   return kOk;
 }
 
@@ -1927,7 +1993,20 @@ int dgbgk_setup_tables(dgbgk_ctx* h, int* iside, double* nx, double* ny, double*
                        double* geome, double* relen, double* mmat, double* rsido, double* vcord,
                        double* sumweight, double* dtec) {
   if (!h) return fail(kErrArg, "null context");
-  copy_tables(h->c.p, iside, nx, ny, el, geome, relen, mmat, rsido, vcord, sumweight, dtec);
+  const Ctx& c = h->c;
+  if (c.eorder.empty()) {
+    copy_tables(c.p, iside, nx, ny, el, geome, relen, mmat, rsido, vcord, sumweight, dtec);
+    return kOk;
+  }
+  // the reference's tables are those of the input numbering
+  Prep q;
+  std::string msg;
+  const Ctx::Input& in = c.in;
+  int st = prepare(c.p.E, c.p.P, c.p.B, in.intma.data(), in.coord.data(), in.bsido.data(),
+                   in.rorder, in.torder, in.eta.data(), in.w.data(), to_params(&in.prm), kSlots,
+                   kSlots, q, msg);
+  if (st) return fail(st, msg);
+  copy_tables(q, iside, nx, ny, el, geome, relen, mmat, rsido, vcord, sumweight, dtec);
   return kOk;
 }
 

@@ -520,4 +520,92 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
   return kOk;
 }
 
+namespace {
+
+// Index along the Hilbert curve of order `bits` of the cell (xi, yi), by the
+// usual quadrant-and-rotate recursion from the top bit down.
+uint64_t hilbert_d(int64_t xi, int64_t yi, int bits) {
+  const int64_t n = int64_t(1) << bits;
+  uint64_t d = 0;
+  for (int64_t s = n >> 1; s > 0; s >>= 1) {
+    const int64_t rx = (xi & s) ? 1 : 0, ry = (yi & s) ? 1 : 0;
+    d += (uint64_t)(s * s) * (uint64_t)((3 * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        xi = n - 1 - xi;
+        yi = n - 1 - yi;
+      }
+      std::swap(xi, yi);
+    }
+  }
+  return d;
+}
+
+}  // namespace
+
+int hilbert_order(int nelem, int npoin, const int* intma, const double* coord,
+                  std::vector<int>& order) {
+  order.clear();
+  if (nelem < 2) return 0;
+  const size_t E = (size_t)nelem;
+  for (size_t k = 0; k < 3 * E; ++k)
+    if (intma[k] < 1 || intma[k] > npoin) return 0;
+  // centroids ((x1 + x2) + x3) / 3 over the input's node order, scaled by
+  // the larger side of their bounding box into 2^16 cells a side
+  std::vector<double> cx(E), cy(E);
+  double lox = 0.0, loy = 0.0, hix = 0.0, hiy = 0.0;
+  for (size_t e = 0; e < E; ++e) {
+    const int* t = &intma[3 * e];
+    const double* p0 = &coord[2 * (size_t)(t[0] - 1)];
+    const double* p1 = &coord[2 * (size_t)(t[1] - 1)];
+    const double* p2 = &coord[2 * (size_t)(t[2] - 1)];
+    cx[e] = (p0[0] + p1[0] + p2[0]) / 3.0;
+    cy[e] = (p0[1] + p1[1] + p2[1]) / 3.0;
+    if (e == 0 || cx[e] < lox) lox = cx[e];
+    if (e == 0 || cy[e] < loy) loy = cy[e];
+    if (e == 0 || cx[e] > hix) hix = cx[e];
+    if (e == 0 || cy[e] > hiy) hiy = cy[e];
+  }
+  if (!(std::isfinite(lox) && std::isfinite(loy) && std::isfinite(hix) && std::isfinite(hiy)))
+    return 0;
+  const int bits = 16;
+  const int64_t n = int64_t(1) << bits;
+  const double span = std::max(std::max(hix - lox, hiy - loy), 1e-300);
+  std::vector<uint64_t> key(E);
+  for (size_t e = 0; e < E; ++e) {
+    const int64_t xi = std::min((int64_t)((cx[e] - lox) / span * (double)(n - 1)), n - 1);
+    const int64_t yi = std::min((int64_t)((cy[e] - loy) / span * (double)(n - 1)), n - 1);
+    key[e] = hilbert_d(xi, yi, bits);
+  }
+  order.resize(E);
+  for (size_t e = 0; e < E; ++e) order[e] = (int)e;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  for (size_t e = 0; e < E; ++e)
+    if (order[e] != (int)e) return 1;
+  order.clear();
+  return 0;
+}
+
+void renumber_elements(int nelem, int nboun, const int* intma, const int* bsido,
+                       const std::vector<int>& order, std::vector<int>& intma_out,
+                       std::vector<int>& bsido_out) {
+  const size_t E = (size_t)nelem;
+  std::vector<int> inv(E);
+  for (size_t s = 0; s < E; ++s) inv[order[s]] = (int)s;
+  intma_out.resize(3 * E);
+  for (size_t s = 0; s < E; ++s)
+    for (int a = 0; a < 3; ++a) intma_out[3 * s + a] = intma[3 * (size_t)order[s] + a];
+  bsido_out.assign(bsido, bsido + 6 * (size_t)std::max(nboun, 0));
+  for (int b = 0; b < nboun; ++b) {
+    int& e = bsido_out[6 * (size_t)b + 2];
+    if (e >= 1 && e <= nelem) e = inv[e - 1] + 1;  // out of range: prepare reports it
+  }
+}
+
+void csr_input_order(Prep& p, const std::vector<int>& order) {
+  for (int q = 0; q < p.P; ++q)
+    std::stable_sort(p.pt_node.begin() + p.pt_ptr[q], p.pt_node.begin() + p.pt_ptr[q + 1],
+                     [&](int a, int b) { return order[a / 3] < order[b / 3]; });
+}
+
 }  // namespace dgbgk

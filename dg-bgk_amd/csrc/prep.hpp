@@ -94,4 +94,24 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
 // The reference's preprocessing steps, exposed for bit-exact checks.
 int getsid(int nelem, int npoin, const int* intma, std::vector<int>& iside);
 
+// Device storage order of the elements: the input elements sorted along a
+// Hilbert curve through their centroids (stable in the input order).  The
+// update kernel's tiles are runs of consecutive elements, so in this order
+// they are compact patches whatever the input numbering.  Returns 1 and
+// fills order (storage position -> input element, 0-based) when the order
+// differs from the input's; 0 (order empty) when it is the input order or a
+// point id is out of range (prepare then reports the mesh error).
+int hilbert_order(int nelem, int npoin, const int* intma, const double* coord,
+                  std::vector<int>& order);
+
+// The mesh renumbered into a storage order: INTMA columns moved, BSIDO(3)
+// (the side's element) renumbered.
+void renumber_elements(int nelem, int nboun, const int* intma, const int* bsido,
+                       const std::vector<int>& order, std::vector<int>& intma_out,
+                       std::vector<int>& bsido_out);
+
+// Point CSR of a renumbered mesh back in input element order, the order in
+// which MACROS accumulates the point averages (src/macros.f90:156-170).
+void csr_input_order(Prep& p, const std::vector<int>& order);
+
 }  // namespace dgbgk

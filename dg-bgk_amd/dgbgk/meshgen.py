@@ -250,6 +250,26 @@ def body_in_box(lx: float, ly: float, cx: float, cy: float, diameter: float,
     return _finish(coord, tri, classify, name, tw=tw, alpha100=alpha100)
 
 
+def renumber_elements(m: Mesh, perm: np.ndarray, name: str | None = None) -> Mesh:
+    """The same mesh with element i being element perm[i] of m: INTMA rows
+    moved, BSIDO's element column (BSIDO(3)) renumbered."""
+    perm = np.asarray(perm, dtype=np.int64)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    bsido = m.bsido.copy()
+    bsido[:, 2] = (inv[m.bsido[:, 2] - 1] + 1).astype(np.int32)
+    return Mesh(intma=np.ascontiguousarray(m.intma[perm]), coord=m.coord, bsido=bsido,
+                name=name or m.name)
+
+
+def shuffle_elements(m: Mesh, seed: int = 0) -> tuple[Mesh, np.ndarray]:
+    """m with its elements in a random order (an input mesh in arbitrary
+    element numbering); returns the mesh and perm (new element i is element
+    perm[i] of m)."""
+    perm = np.random.default_rng(seed).permutation(m.nelem)
+    return renumber_elements(m, perm, f"{m.name}-shuffled{seed}"), perm
+
+
 def mesh_quality(m: Mesh) -> dict:
     p = m.coord[m.intma - 1]
     a = np.linalg.norm(p[:, 1] - p[:, 0], axis=1)

@@ -35,6 +35,14 @@ def cylinder_lobatto10(elems=1500, alpha=0.9):
     return cases.Case("cylinder_lobatto10", c.mesh, eta, w, 10, prm, nsteps=10)
 
 
+def shuffled(case, seed=0):
+    """case on its mesh with the elements in a random order; returns the case
+    and perm (element i of the new mesh is element perm[i] of case's)."""
+    m, perm = meshgen.shuffle_elements(case.mesh, seed)
+    return cases.Case(case.name + "-shuffled", m, case.eta, case.w, case.torder,
+                      dict(case.prm), nsteps=case.nsteps), perm
+
+
 def rel_l2(a, b):
     a = np.asarray(a, float).ravel()
     b = np.asarray(b, float).ravel()

@@ -7,7 +7,7 @@ import re
 import numpy as np
 import pytest
 
-from small_cases import c1, cylinder_lobatto10, cylinder_small, lobatto10_walls
+from small_cases import c1, cylinder_lobatto10, cylinder_small, lobatto10_walls, shuffled
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -22,9 +22,15 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, name), name
 
 
+def _cylinder_shuffled():
+    return shuffled(cylinder_small(), seed=3)[0]
+
+
 @pytest.mark.parametrize("make,lookup", [(c1, 1), (cylinder_small, 1), (lobatto10_walls, 0),
-                                         (cylinder_lobatto10, 0)])
+                                         (cylinder_lobatto10, 0), (_cylinder_shuffled, 1)])
 def test_setup_tables_match_oracle_bit_for_bit(make, lookup):
+    """The reference's tables in the input element numbering, also where the
+    library keeps the elements in another order."""
     from dgbgk.solver import Setup
     from oracle_py import OracleRun
     case = make()
@@ -49,6 +55,21 @@ def test_tiles_cover_every_element_once():
         assert s.ntiles >= (case.mesh.nelem + 255) // 256
         # halo slots take at most a third of a tile on these meshes
         assert s.ntiles <= (case.mesh.nelem + 169) // 170
+
+
+def test_tiles_do_not_depend_on_the_input_numbering(monkeypatch):
+    """The library keeps elements in Hilbert order of their centroids, so a
+    mesh in random element order gets the same compact tiles as the mesh
+    generator's own (Hilbert) order; kept in the input order
+    (DGBGK_REORDER=0) its tiles would hold mostly halo elements."""
+    from dgbgk.solver import Setup
+    case = cylinder_small(elems=6000)
+    sh, _ = shuffled(case, seed=1)
+    n0 = Setup(case).ntiles
+    assert Setup(sh).ntiles == n0
+    monkeypatch.setenv("DGBGK_REORDER", "0")
+    assert Setup(case).ntiles == n0          # the generator's order is kept as is
+    assert Setup(sh).ntiles > 2 * n0
 
 
 def _err(case, **kw):
