@@ -83,7 +83,10 @@ typedef struct dgbgk_ctx dgbgk_ctx;
  * VSPACE, ALOTIM (src/dg_lin_convec.f90:96-123, src/spectralbgk.f90:205,
  * src/advnce.f90:133), the INICON2 Maxwellian seed (src/inicon2.f90:120-193)
  * and the initial MACROS (src/advnce.f90:139-160).  eta/w are the radial
- * quadrature (the Lobatto file's two columns, src/vspace.f90:52-61). */
+ * quadrature (the Lobatto file's two columns, src/vspace.f90:52-61).
+ * Every per-element array of this interface is in the caller's element
+ * numbering; the device stores elements in Hilbert order of their
+ * centroids (DGBGK_REORDER=0 in the environment keeps the input order). */
 int dgbgk_create(int nelem, int npoin, int nboun, const int *intma,
                  const double *coord, const int *bsido, int rorder, int torder,
                  const double *eta, const double *w, const dgbgk_params *prm,
