@@ -90,3 +90,20 @@ def test_set_f_get_f_round_trip_in_input_numbering(prec):
             assert np.array_equal(nb[k], na[k][perm]), k
         a.close()
         b.close()
+
+
+def test_context_setup_tables_are_the_input_meshs():
+    """dgbgk_setup_tables on a context whose elements the library renumbered
+    gives the reference's tables of the input mesh, bit for bit."""
+    from dgbgk.solver import Solver
+    from oracle_py import OracleRun
+    case, _ = shuffled(cylinder_small(), seed=4)
+    print(f"case cylinder E={case.mesh.nelem} shuffled, context setup tables", flush=True)
+    s, o = Solver(case), OracleRun(case)
+    t, ot = s.tables(), o.tables()
+    assert np.array_equal(t["iside"], o.iside())
+    assert np.array_equal(t["vcord"], o.vcord())
+    assert t["sumweight"] == o.sumweight()
+    for k in ("nx", "ny", "el", "geome", "relen", "mmat", "rsido", "dtec"):
+        assert np.array_equal(t[k], ot[k]), k
+    s.close()
