@@ -1647,8 +1647,10 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
     im = im_r.data();
     bs = bs_r.data();
   }
-  int st = prepare(nelem, npoin, nboun, im, coord, bs, rorder, torder, eta, w,
-                   to_params(prm), c.slots, owned_cap, c.p, msg);
+  Params dprm = to_params(prm);
+  if (const char* ta = std::getenv("DGBGK_TILE_ALIGN")) dprm.tile_align = std::max(1, std::atoi(ta));
+  int st = prepare(nelem, npoin, nboun, im, coord, bs, rorder, torder, eta, w, dprm, c.slots,
+                   owned_cap, c.p, msg);
   if (st) {
     delete h;
     return fail(st, msg);

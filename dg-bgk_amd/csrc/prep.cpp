@@ -493,6 +493,12 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
         ++e1;
       }
       if (e1 == e0) { msg = "tile construction failed"; return kErrTile; }
+      // owned counts a multiple of tile_align (all tiles but the last), so
+      // tile boundaries fall on cache-line boundaries of every f row; the
+      // elements given back can only replace halo entries, so the slots
+      // still suffice
+      if (prm.tile_align > 1 && e1 < E && e1 - e0 >= prm.tile_align)
+        e1 = e0 + (e1 - e0) / prm.tile_align * prm.tile_align;
       int nown = e1 - e0;
       // collect halo elements (marked, outside the owned range)
       int hoff = (int)p.halo_elems.size(), nhal = 0;

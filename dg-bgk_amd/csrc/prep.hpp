@@ -34,6 +34,7 @@ struct Params {
   double t1, p1, u0, v0;
   double cinf[4];
   int inf, immat, precision, lookup, order, vchunk;
+  int tile_align = 1;  // update tiles: owned elements a multiple of this
 };
 
 enum EdgeKind : int { kInterior = 0, kInflow = 1, kOutflow = 2, kWall = 3 };
