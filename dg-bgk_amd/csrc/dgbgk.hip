@@ -39,16 +39,6 @@ __device__ __forceinline__ int xcd_logical(int bid, int nblocks) {
   return xcd * q + min(xcd, r) + loc;
 }
 
-// An element's fixed dither phase in [0, 1): an integer hash of its index
-// (lowbias32 mixing), the same on every device and every run.
-__device__ __forceinline__ float node_phase(unsigned x) {
-  x ^= x >> 16;
-  x *= 0x7feb352du;
-  x ^= x >> 15;
-  x *= 0x846ca68bu;
-  x ^= x >> 16;
-  return (float)(x >> 8) * 0x1p-24f;
-}
 
 // Unbiased rounding of an FP32 store f + d: d gains a dither uniform over
 // +-1/2 ulp of f before the one rounding, so the stored value rounds up
@@ -389,7 +379,7 @@ k_update(UpdArgs<T> a) {
     }
     s.mask = a.emask[e];
     s.wsl = a.ewslot[e];
-    s.dph = node_phase((unsigned)e);
+    s.dph = a.ephase[e];
   }
   T res = 0, chg = 0;
 
@@ -513,7 +503,7 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
     const int mask = a.emask[e];
     const T alpha = a.alpha;
     // FP32 store dither (as in k_update): node, ordinate and step phases
-    const float wph = node_phase((unsigned)e);
+    const float wph = a.ephase[e];
     const double stw = a.scal[6] * 0.6180339887498949;
     const float sph = (float)(stw - floor(stw));
     for (int v = vb; v < ve; ++v) {
@@ -1093,6 +1083,7 @@ struct Ctx {
          *pts = nullptr, *ps_glob = nullptr, *coord = nullptr, *res_part = nullptr,
          *mu_part = nullptr, *mass_part = nullptr, *scal = nullptr, *ps4 = nullptr,
          *msum = nullptr;
+  float* ephase = nullptr;  // [E] FP32 store dither phase per element
   StepRec* rec = nullptr;
   int rec_cap = 0;
   int wall_tpb = 256;
@@ -1264,6 +1255,7 @@ int step_once(Ctx& c) {
   u.tiles = c.tiles; u.halo = c.halo; u.wpart = (T*)c.wpart; u.vtd = c.vtd; u.msum = c.msum;
   u.scal = c.scal; u.res_part = c.res_part; u.res_stride = c.nblk_upd + c.nblk_wall;
   u.E = p.E; u.V = p.V; u.ntiles = c.ntiles; u.vchunk = c.vchunk;
+  u.ephase = c.ephase;
   u.ldf = (long long)c.ldf;
   u.dither = c.dither;
   tbegin(c, 0);
@@ -1277,6 +1269,7 @@ int step_once(Ctx& c) {
     w.emask = c.emask; w.vtab = (const T*)c.vtab; w.refl_i = c.refl_i;
     w.refl_t = (const T*)c.refl_t; w.wall_bside = c.wall_bside; w.eta = c.eta;
     w.wall_elems = c.wall_elems; w.wpart = (const T*)c.wpart; w.scal = c.scal;
+    w.ephase = c.ephase;
     w.res_part = c.res_part + c.nblk_upd; w.res_stride = c.nblk_upd + c.nblk_wall;
     w.alpha = (T)p.prm.alpha; w.E = p.E; w.V = p.V;
     w.ldf = (long long)c.ldf;
@@ -1351,11 +1344,46 @@ int run_steps(Ctx& c, int n) {
   return kOk;
 }
 
+// An element's fixed dither phase in [0, 1) for the FP32 store: an integer
+// hash (lowbias32 mixing) of the bits of its centroid, so the same on every
+// device and run, whatever the element numbering or storage order.
+uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+float element_phase(const Prep& p, int e) {
+  const int* t = &p.intma[3 * (size_t)e];
+  double cx = 0.0, cy = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    cx += p.coord[2 * (size_t)(t[a] - 1)];
+    cy += p.coord[2 * (size_t)(t[a] - 1) + 1];
+  }
+  uint64_t bx, by;
+  std::memcpy(&bx, &cx, 8);
+  std::memcpy(&by, &cy, 8);
+  uint32_t h = mix32((uint32_t)by ^ 0x9e3779b9u);
+  h = mix32(h ^ (uint32_t)(by >> 32));
+  h = mix32(h ^ (uint32_t)bx);
+  h = mix32(h ^ (uint32_t)(bx >> 32));
+  return (float)(h >> 8) * 0x1p-24f;
+}
+
 template <typename T>
 int upload_tables(Ctx& c) {
   const Prep& p = c.p;
   const int E = p.E, V = p.V;
   const size_t N3 = 3 * (size_t)E;
+  {
+    std::vector<float> ph(E);
+    for (int e = 0; e < E; ++e) ph[e] = element_phase(p, e);
+    int st = upload(&c.ephase, ph, c.owned);
+    if (st) return st;
+  }
   std::vector<T> geo(8 * (size_t)E), enx(3 * (size_t)E), eny(3 * (size_t)E), eal(3 * (size_t)E);
   std::vector<int> einfo(3 * (size_t)E), eslot(3 * (size_t)E), emask(E), ewslot(E);
   std::vector<T> unx(3 * (size_t)E), uny(3 * (size_t)E);
@@ -1654,6 +1682,27 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   if (st) {
     delete h;
     return fail(st, msg);
+  }
+  // inside each tile, the elements that other tiles read as halo last,
+  // grouped by reader; the same tiles are rebuilt on that order
+  const char* bo = std::getenv("DGBGK_BORDER_ORDER");
+  if (reorder_enabled() && !(bo && bo[0] == '0')) {
+    std::vector<int> perm, bounds;
+    border_order(c.p, perm, bounds);
+    bool moved = false;
+    for (int i = 0; i < nelem && !moved; ++i) moved = perm[i] != i;
+    if (moved) {
+      std::vector<int> fin(nelem);
+      for (int i = 0; i < nelem; ++i) fin[i] = c.eorder.empty() ? perm[i] : c.eorder[perm[i]];
+      c.eorder.swap(fin);
+      renumber_elements(nelem, nboun, intma, bsido, c.eorder, im_r, bs_r);
+      st = prepare(nelem, npoin, nboun, im_r.data(), coord, bs_r.data(), rorder, torder, eta, w,
+                   dprm, c.slots, owned_cap, c.p, msg, &bounds);
+      if (st) {
+        delete h;
+        return fail(st, msg);
+      }
+    }
   }
   if (!c.eorder.empty()) {
     csr_input_order(c.p, c.eorder);

@@ -61,6 +61,7 @@ struct UpdArgs {
   double* __restrict__ msum;       // [nvchunks][3][3E] moment partials of f^{k+1}
   const double* __restrict__ scal; // [0] = DTE
   double* __restrict__ res_part;   // per block: max|increment|; +res_stride: max|change|
+  const float* __restrict__ ephase; // [E] FP32 store dither phase per element
   int res_stride;
   int E, V, ntiles, vchunk;
   long long ldf;  // row stride of f
@@ -87,6 +88,7 @@ struct WallArgs {
   const T* __restrict__ wpart;
   const double* __restrict__ scal;
   double* __restrict__ res_part;
+  const float* __restrict__ ephase; // [E] FP32 store dither phase per element
   int res_stride;
   T alpha;
   int E, V, T_, ngroups, gsize, nwe;  // ngroups threads per wall element, gsize ordinates each

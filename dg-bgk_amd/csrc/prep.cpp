@@ -159,7 +159,7 @@ uint64_t pair_key(int a, int b) {
 int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coord,
             const int* bsido, int rorder, int torder, const double* eta,
             const double* w, const Params& prm, int tile_slots, int tile_owned, Prep& p,
-            std::string& msg) {
+            std::string& msg, const std::vector<int>* tile_bounds) {
   if (nelem < 1 || npoin < 3 || nboun < 0 || rorder < 1 || torder < 1) {
     msg = "invalid sizes";
     return kErrArg;
@@ -463,19 +463,27 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
   }
   // tiles: contiguous owned ranges of at most tile_owned elements whose
   // owned + halo elements fit the slots (tile_owned a multiple of the
-  // wavefront size keeps halo slots out of the owned wavefronts)
+  // wavefront size keeps halo slots out of the owned wavefronts), or the
+  // ranges given by tile_bounds (tile starts, then E)
   {
     if (tile_owned <= 0 || tile_owned > tile_slots) tile_owned = tile_slots;
+    const bool forced = tile_bounds && !tile_bounds->empty();
+    if (forced && (tile_bounds->front() != 0 || tile_bounds->back() != E)) {
+      msg = "tile bounds do not cover the mesh";
+      return kErrTile;
+    }
     p.nbr_slot.assign(3 * (size_t)E, -1);
     std::vector<int> mark(E, -1), hidx(E, -1);
     int e0 = 0;
     int tile = 0;
-    std::vector<int> halo;
     while (e0 < E) {
-      halo.clear();
       int nh = 0, e1 = e0;
+      if (forced) {
+        e1 = tile + 1 < (int)tile_bounds->size() ? (*tile_bounds)[tile + 1] : e0;
+        if (e1 <= e0 || e1 > E) { msg = "tile bounds out of order"; return kErrTile; }
+      }
       // grow [e0, e1)
-      while (e1 < E) {
+      while (!forced && e1 < E) {
         // tentative halo after adding e1
         int add = 0;
         for (int k = 0; k < 3; ++k) {
@@ -497,22 +505,30 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
       // tile boundaries fall on cache-line boundaries of every f row; the
       // elements given back can only replace halo entries, so the slots
       // still suffice
-      if (prm.tile_align > 1 && e1 < E && e1 - e0 >= prm.tile_align)
+      if (!forced && prm.tile_align > 1 && e1 < E && e1 - e0 >= prm.tile_align)
         e1 = e0 + (e1 - e0) / prm.tile_align * prm.tile_align;
       int nown = e1 - e0;
-      // collect halo elements (marked, outside the owned range)
+      // collect halo elements (neighbours outside the owned range), then
+      // give them slots in storage order: halo slots of one wavefront then
+      // load runs of consecutive elements
       int hoff = (int)p.halo_elems.size(), nhal = 0;
       for (int e = e0; e < e1; ++e)
         for (int k = 0; k < 3; ++k) {
           int nb = p.edges[3 * (size_t)e + k].nbr;
-          if (nb < 0) continue;
-          if (nb >= e0 && nb < e1) { p.nbr_slot[3 * (size_t)e + k] = nb - e0; continue; }
+          if (nb < 0 || (nb >= e0 && nb < e1)) continue;
           if (hidx[nb] < hoff || hidx[nb] >= hoff + nhal || p.halo_elems[hidx[nb]] != nb) {
             hidx[nb] = hoff + nhal;
             p.halo_elems.push_back(nb);
             ++nhal;
           }
-          p.nbr_slot[3 * (size_t)e + k] = nown + (hidx[nb] - hoff);
+        }
+      std::sort(p.halo_elems.begin() + hoff, p.halo_elems.end());
+      for (int j = 0; j < nhal; ++j) hidx[p.halo_elems[hoff + j]] = hoff + j;
+      for (int e = e0; e < e1; ++e)
+        for (int k = 0; k < 3; ++k) {
+          int nb = p.edges[3 * (size_t)e + k].nbr;
+          if (nb < 0) continue;
+          p.nbr_slot[3 * (size_t)e + k] = (nb >= e0 && nb < e1) ? nb - e0 : nown + (hidx[nb] - hoff);
         }
       if (nown + nhal > tile_slots) { msg = "tile overflow"; return kErrTile; }
       p.tile_e0.push_back(e0);
@@ -606,6 +622,23 @@ void renumber_elements(int nelem, int nboun, const int* intma, const int* bsido,
     int& e = bsido_out[6 * (size_t)b + 2];
     if (e >= 1 && e <= nelem) e = inv[e - 1] + 1;  // out of range: prepare reports it
   }
+}
+
+void border_order(const Prep& p, std::vector<int>& perm, std::vector<int>& bounds) {
+  const int E = p.E, nt = (int)p.tile_e0.size();
+  // the lowest tile that reads each element as halo, or -1
+  std::vector<int> reader(E, -1);
+  for (int t = nt - 1; t >= 0; --t)
+    for (int j = 0; j < p.tile_nhalo[t]; ++j) reader[p.halo_elems[p.tile_hoff[t] + j]] = t;
+  perm.resize(E);
+  for (int e = 0; e < E; ++e) perm[e] = e;
+  for (int t = 0; t < nt; ++t) {
+    const int e0 = p.tile_e0[t], e1 = e0 + p.tile_nown[t];
+    std::stable_sort(perm.begin() + e0, perm.begin() + e1,
+                     [&](int a, int b) { return reader[a] < reader[b]; });
+  }
+  bounds.assign(p.tile_e0.begin(), p.tile_e0.end());
+  bounds.push_back(E);
 }
 
 void csr_input_order(Prep& p, const std::vector<int>& order) {

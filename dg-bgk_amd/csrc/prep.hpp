@@ -90,7 +90,7 @@ struct Prep {
 int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coord,
             const int* bsido, int rorder, int torder, const double* eta,
             const double* w, const Params& prm, int tile_slots, int tile_owned, Prep& out,
-            std::string& msg);
+            std::string& msg, const std::vector<int>* tile_bounds = nullptr);
 
 // The reference's preprocessing steps, exposed for bit-exact checks.
 int getsid(int nelem, int npoin, const int* intma, std::vector<int>& iside);
@@ -110,6 +110,15 @@ int hilbert_order(int nelem, int npoin, const int* intma, const double* coord,
 void renumber_elements(int nelem, int nboun, const int* intma, const int* bsido,
                        const std::vector<int>& order, std::vector<int>& intma_out,
                        std::vector<int>& bsido_out);
+
+// Refinement of a storage order whose tiles are built (p): inside each
+// tile's owned range, the elements no other tile reads first, then those
+// that other tiles read as halo, grouped by the lowest such tile, each group
+// in storage order.  A tile's halo then reads runs of consecutive elements
+// rather than elements scattered over its neighbours' ranges.  perm maps a
+// refined position to a position of p's order; bounds are the tile starts
+// and E, for prepare to rebuild the same tiles.
+void border_order(const Prep& p, std::vector<int>& perm, std::vector<int>& bounds);
 
 // Point CSR of a renumbered mesh back in input element order, the order in
 // which MACROS accumulates the point averages (src/macros.f90:156-170).
