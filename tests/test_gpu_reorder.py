@@ -107,3 +107,24 @@ def test_context_setup_tables_are_the_input_meshs():
     for k in ("nx", "ny", "el", "geome", "relen", "mmat", "rsido", "dtec"):
         assert np.array_equal(t[k], ot[k]), k
     s.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_border_grouping_is_bit_identical(prec, monkeypatch):
+    """Grouping tile border elements changes only where elements are stored
+    inside the same tiles: f and the node fields are bit-identical with it
+    and without it (DGBGK_BORDER_ORDER=0), in both forms (the FP32 store's
+    dither phase belongs to the element, not to its storage position)."""
+    case = cylinder_small()
+    print(f"case cylinder E={case.mesh.nelem} border grouping on/off, precision {prec}",
+          flush=True)
+    a = _run(case, prec, 70)
+    monkeypatch.setenv("DGBGK_BORDER_ORDER", "0")
+    b = _run(case, prec, 70)
+    assert a["tiles"] == b["tiles"]
+    assert np.array_equal(a["f"], b["f"])
+    for k in ("disnd", "disux", "disuy", "disps", "mu"):
+        assert np.array_equal(a["nf"][k], b["nf"][k]), k
+    for ra, rb in zip(a["recs"], b["recs"]):
+        for k in ("sumres", "dte", "change"):
+            assert ra[k] == rb[k], k
