@@ -916,15 +916,41 @@ __global__ void __launch_bounds__(1024) k_final(const double* __restrict__ res_p
                                                double* __restrict__ scal,
                                                StepRec* __restrict__ rec, int rec_cap,
                                                int record) {
-  __shared__ double s_red[16];
+  __shared__ double s_red[6][16];
   double r = 0.0, rc = 0.0, m = 0.0, ms = 0.0, lift = 0.0, drag = 0.0;
-  for (int i = threadIdx.x; i < nres; i += blockDim.x) {
-    r = fmax(r, res_part[i]);
-    rc = fmax(rc, res_part[nres + i]);
+  // each thread's partials, up to kLd per array loaded before any is
+  // combined (one memory latency instead of one per partial); combined in
+  // index order, so the mass sum keeps its order (out-of-range slots add
+  // 0 and all partials are >= 0)
+  constexpr int kLd = 8;
+  const int nt = blockDim.x;
+  for (int b = threadIdx.x; b < nres; b += kLd * nt) {
+    double x[kLd], y[kLd];
+#pragma unroll
+    for (int j = 0; j < kLd; ++j) {
+      const int i = b + j * nt;
+      x[j] = i < nres ? res_part[i] : 0.0;
+      y[j] = i < nres ? res_part[nres + i] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kLd; ++j) {
+      r = fmax(r, x[j]);
+      rc = fmax(rc, y[j]);
+    }
   }
-  for (int i = threadIdx.x; i < nmom; i += blockDim.x) {
-    m = fmax(m, mu_part[i]);
-    ms += mass_part[i];
+  for (int b = threadIdx.x; b < nmom; b += kLd * nt) {
+    double x[kLd], y[kLd];
+#pragma unroll
+    for (int j = 0; j < kLd; ++j) {
+      const int i = b + j * nt;
+      x[j] = i < nmom ? mu_part[i] : 0.0;
+      y[j] = i < nmom ? mass_part[i] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kLd; ++j) {
+      m = fmax(m, x[j]);
+      ms += y[j];
+    }
   }
   for (int i = threadIdx.x; i < nf; i += blockDim.x) {
     const int p1 = fside[2 * i], p2 = fside[2 * i + 1];
@@ -937,13 +963,41 @@ __global__ void __launch_bounds__(1024) k_final(const double* __restrict__ res_p
     lift -= psav * len * (py / mag);
     drag -= psav * len * (px / mag);
   }
-  r = block_max(r, s_red);
-  rc = block_max(rc, s_red);
-  m = block_max(m, s_red);
-  ms = block_sum(ms, s_red);
-  lift = block_sum(lift, s_red);
-  drag = block_sum(drag, s_red);
+  // the three maxima and three sums in one LDS round: each wavefront
+  // reduces its six values, then thread j combines value j over the
+  // wavefronts in wavefront order (the order of block_max / block_sum)
+  for (int o = 32; o > 0; o >>= 1) {
+    r = fmax(r, __shfl_down(r, o, 64));
+    rc = fmax(rc, __shfl_down(rc, o, 64));
+    m = fmax(m, __shfl_down(m, o, 64));
+    ms += __shfl_down(ms, o, 64);
+    lift += __shfl_down(lift, o, 64);
+    drag += __shfl_down(drag, o, 64);
+  }
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_red[0][w] = r;
+    s_red[1][w] = rc;
+    s_red[2][w] = m;
+    s_red[3][w] = ms;
+    s_red[4][w] = lift;
+    s_red[5][w] = drag;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int j = threadIdx.x;
+    double x = s_red[j][0];
+    for (int i = 1; i < nw; ++i) x = j < 3 ? fmax(x, s_red[j][i]) : x + s_red[j][i];
+    s_red[j][0] = x;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    r = s_red[0][0];
+    rc = s_red[1][0];
+    m = s_red[2][0];
+    ms = s_red[3][0];
+    lift = s_red[4][0];
+    drag = s_red[5][0];
     if (record >= 0) {
       // scal[6] counts completed steps (ITIME of src/advnce.f90:220)
       const long long itime = (long long)scal[6] + 1;
