@@ -858,19 +858,15 @@ __global__ void __launch_bounds__(256) k_eta(const T* __restrict__ f,
   if (threadIdx.x == 0) eta[b] = s / bottom;
 }
 
-// MACROS point averages (src/macros.f90:156-188) over the point CSR.
-__global__ void __launch_bounds__(256) k_points(const double* __restrict__ mom,
-                                                const double* __restrict__ rj,
-                                                const int* __restrict__ ptr,
-                                                const int* __restrict__ node,
-                                                double* __restrict__ pts,
-                                                double* __restrict__ ps_glob, int E,
-                                                int P, double mm, double rgas,
-                                                int gather, const double* __restrict__ scal) {
-  // in a step, gather only on the reference's GETMAC steps (every 1000)
-  if (scal != nullptr && (((long long)scal[6] + 1) % 1000) != 0) return;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= P) return;
+// MACROS point averages (src/macros.f90:156-188) of point q over the point
+// CSR; gather: also GETMAC's copy of the pressure for GETFOR.
+__device__ __forceinline__ void point_average(int q, const double* __restrict__ mom,
+                                              const double* __restrict__ rj,
+                                              const int* __restrict__ ptr,
+                                              const int* __restrict__ node,
+                                              double* __restrict__ pts,
+                                              double* __restrict__ ps_glob, int E, int P,
+                                              double mm, double rgas, int gather) {
   const size_t N3 = 3 * (size_t)E;
   double flag = 0, nd = 0, u = 0, vv = 0, ps = 0;
   for (int i = ptr[q]; i < ptr[q + 1]; ++i) {
@@ -905,6 +901,27 @@ __global__ void __launch_bounds__(256) k_points(const double* __restrict__ mom,
   if (gather) ps_glob[q] = ps;
 }
 
+__global__ void __launch_bounds__(256) k_points(const double* __restrict__ mom,
+                                                const double* __restrict__ rj,
+                                                const int* __restrict__ ptr,
+                                                const int* __restrict__ node,
+                                                double* __restrict__ pts,
+                                                double* __restrict__ ps_glob, int E,
+                                                int P, double mm, double rgas, int gather) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < P) point_average(q, mom, rj, ptr, node, pts, ps_glob, E, P, mm, rgas, gather);
+}
+
+struct PointArgs {
+  const double* mom;
+  const double* rj;
+  const int* ptr;
+  const int* node;
+  double* pts;
+  int E, P;
+  double mm, rgas;
+};
+
 // Step epilogue: GETRES max, mass, GETFOR (src/getfor.f90:29-49) on the
 // gathered point pressure, the residual record, and ALOTIM2's next DTE.
 __global__ void __launch_bounds__(1024) k_final(const double* __restrict__ res_part, int nres,
@@ -912,11 +929,12 @@ __global__ void __launch_bounds__(1024) k_final(const double* __restrict__ res_p
                                                const double* __restrict__ mass_part, int nmom,
                                                const int* __restrict__ fside, int nf,
                                                const double* __restrict__ coord,
-                                               const double* __restrict__ ps_glob,
+                                               double* __restrict__ ps_glob,
                                                double* __restrict__ scal,
                                                StepRec* __restrict__ rec, int rec_cap,
-                                               int record) {
+                                               int record, PointArgs pa) {
   __shared__ double s_red[6][16];
+  const double steps_done = scal[6];  // in flight with the partials below
   double r = 0.0, rc = 0.0, m = 0.0, ms = 0.0, lift = 0.0, drag = 0.0;
   // each thread's partials, up to kLd per array loaded before any is
   // combined (one memory latency instead of one per partial); combined in
@@ -951,6 +969,15 @@ __global__ void __launch_bounds__(1024) k_final(const double* __restrict__ res_p
       m = fmax(m, x[j]);
       ms += y[j];
     }
+  }
+  // In a step, GETMAC on every 1000th step (src/advnce.f90:403-406): the
+  // point averages and the gathered pressure that GETFOR below uses.  Rare,
+  // so this block does it rather than a launch every step.
+  if (record >= 0 && (((long long)steps_done + 1) % 1000) == 0) {
+    for (int q = threadIdx.x; q < pa.P; q += blockDim.x)
+      point_average(q, pa.mom, pa.rj, pa.ptr, pa.node, pa.pts, ps_glob, pa.E, pa.P, pa.mm,
+                    pa.rgas, 1);
+    __syncthreads();
   }
   for (int i = threadIdx.x; i < nf; i += blockDim.x) {
     const int p1 = fside[2 * i], p2 = fside[2 * i + 1];
@@ -1197,19 +1224,21 @@ int launch_moments(Ctx& c, bool partials = false) {
 }
 
 int launch_final(Ctx& c, int record) {
+  PointArgs pa{c.mom, c.rj, c.pt_ptr, c.pt_node, c.pts, c.p.E, c.p.P, c.p.prm.m / kNa,
+               c.p.prm.rgas};
   hipLaunchKernelGGL(k_final, dim3(1), dim3(1024), 0, c.stream, c.res_part,
                      c.nblk_upd + c.nblk_wall, c.mu_part, c.mass_part, c.nblk_mom, c.fside,
-                     c.nforce, c.coord, c.ps_glob, c.scal, c.rec, c.rec_cap, record);
+                     c.nforce, c.coord, c.ps_glob, c.scal, c.rec, c.rec_cap, record, pa);
   HIPCK(hipGetLastError());
   return kOk;
 }
 
 // in_step: gate on the device step counter (GETMAC every 1000 steps)
-int launch_points(Ctx& c, int gather, bool in_step = false) {
+int launch_points(Ctx& c, int gather) {
   const int P = c.p.P;
   hipLaunchKernelGGL(k_points, dim3((P + 255) / 256), dim3(256), 0, c.stream, c.mom, c.rj,
                      c.pt_ptr, c.pt_node, c.pts, c.ps_glob, c.p.E, P, c.p.prm.m / kNa,
-                     c.p.prm.rgas, gather, in_step ? (const double*)c.scal : nullptr);
+                     c.p.prm.rgas, gather);
   HIPCK(hipGetLastError());
   return kOk;
 }
@@ -1343,12 +1372,8 @@ int step_once(Ctx& c) {
   int st = launch_moments<T>(c, c.fuse_mom != 0);
   if (st) return st;
   tend(c, 2);
-  tbegin(c, 5);
-  st = launch_points(c, 1, true);
-  if (st) return st;
-  tend(c, 5);
   tbegin(c, 4);
-  st = launch_final(c, 0);
+  st = launch_final(c, 0);  // with GETMAC's point gather every 1000th step
   tend(c, 4);
   return st;
 }

@@ -391,3 +391,32 @@ def test_unit_square_2000_steps():
             assert errs["u,v"] < 5e-3
             assert drift < 1e-3
         s.close()
+
+
+def test_getmac_step_gathers_the_point_pressure():
+    """GETMAC runs every 1000th step (src/advnce.f90:403-406) and GETFOR every
+    step on its pressure (src/advnce.f90:430-431): the 1000th step's
+    record takes lift and drag from the point pressure gathered in the step
+    itself, the same as a fresh GETFOR on the state after the step; the
+    steps between keep the last gathered pressure.  Checked with step 1000
+    launched on its own and inside a replayed 64-step graph."""
+    from dgbgk.solver import Solver
+    case = cylinder_small(elems=800)
+    print(f"case cylinder E={case.mesh.nelem} 1000 steps, GETMAC step", flush=True)
+    s = Solver(case)
+    rs = s.step(1000)  # 15 graphs of 64 steps, then 40 steps launched one by one
+    lift, drag = s.forces()
+    s.close()
+    a = rs[999]
+    assert abs(a["drag"] - drag) <= 1e-12 * max(abs(drag), 1.0)
+    assert abs(a["lift"] - lift) <= 1e-12 * max(abs(drag), 1.0)
+    assert rs[998]["drag"] == rs[0]["drag"]  # stale since the last gather
+    assert rs[998]["drag"] != a["drag"]
+    g = Solver(case)
+    g.step(8)
+    rg = g.step(1024)  # steps 9..1032, all in graph replays
+    g.close()
+    b = rg[1000 - 9]
+    assert b["drag"] == a["drag"] and b["lift"] == a["lift"]
+    assert rg[999 - 9]["drag"] == rs[998]["drag"]
+    assert all(r["drag"] == b["drag"] for r in rg[1000 - 9:])
