@@ -142,3 +142,26 @@ def lobatto_case(mesh: meshgen.Mesh, eta, w, torder: int, alpha: float = 0.9,
     prm = base_params(u0=u0, kn=kn, length=1.0, rv=rv, alpha=alpha)
     return Case("custom", mesh, np.asarray(eta, float), np.asarray(w, float),
                 torder, prm)
+
+
+def from_run_dir(directory: str, deck: str = "run.con", single: bool = False) -> Case:
+    """The Case a reference run directory describes: its deck's keys, its
+    mesh file and its velocity file.  single=True rounds every real input
+    to REAL(4), as the reference (default REAL) reads them."""
+    import os
+
+    from . import io
+    keys = io.read_deck(os.path.join(directory, deck))
+    m = io.read_plt(os.path.join(directory, keys["PSPACEFILE"]))
+    eta, w = io.read_lobatto(os.path.join(directory, keys["LOBATTOFILE"]))
+    r = (lambda x: float(np.float32(x))) if single else float
+    if single:
+        m.coord = m.coord.astype(np.float32).astype(np.float64)
+        eta = eta.astype(np.float32).astype(np.float64)
+        w = w.astype(np.float32).astype(np.float64)
+    prm = dict(rv=r(keys["RV"]), csafm=r(keys["CSAFM"]), alpha=r(keys["ALPHA"]),
+               rgas=r(keys["R"]), d=r(keys["D"]), m=r(keys["M"]), t1=r(keys["T1"]),
+               p1=r(keys["P1"]), u0=r(keys["U0"]), v0=r(keys["V0"]),
+               cinf=[r(x) for x in keys["CINF"]], inf=int(keys["INF"]))
+    return Case(os.path.basename(os.path.normpath(directory)), m, eta, w,
+                int(keys["TORDER"]), prm, nsteps=int(keys["NTIME"]))

@@ -167,6 +167,26 @@ def results_block(itime: int, a, b, c) -> str:
     return "\n".join(out) + "\n"
 
 
+def read_residual(path: str) -> np.ndarray:
+    """RESIDUAL.RES: one row per step, ITIME LOG(SUMRES/SRITM1) [LIFT DRAG]
+    (src/advnce.f90:427-432, list-directed)."""
+    rows = [[_fnum(t) for t in line.split()] for line in open(path) if line.strip()]
+    return np.array(rows, dtype=np.float64)
+
+
+def read_results(path: str) -> dict:
+    """RESULTS1.RES / RESULTS2.RES: {ITIME: (NPOIN, 3) array} of the blocks
+    written at output steps (src/advnce.f90:163-170,405-416,466-470)."""
+    blocks, cur = {}, None
+    for line in open(path):
+        if line.startswith("RESULTS"):
+            cur = int(line.split()[2])
+            blocks[cur] = []
+        elif line.strip():
+            blocks[cur].append([_fnum(t) for t in line.split()[1:4]])
+    return {k: np.array(v, dtype=np.float64) for k, v in blocks.items()}
+
+
 def _ordinate_xy(eta, torder: int, rv: float):
     """Cartesian velocity of every ordinate, ring-major, as the reference's
     restart writer maps VCORD back (src/output.f90:112-118, PI = 3.1416)."""
@@ -239,10 +259,14 @@ def write_run_dir(case, directory: str, ntime: int, forceout: int = 1,
                   deck_name: str = "run.con", restart_in: str | None = None) -> str:
     """Writes deck + mesh + velocity file of a Case as a reference-style run
     directory; returns the deck path.  restart_in: start from that restart
-    file (IVD%RS = 1) instead of the INICON2 state."""
+    file (IVD%RS = 1) instead of the INICON2 state.  The nodal partition
+    file names one partition for every point (all zeros), as a one-rank run
+    of the reference reads it (src/ptmesh.f90:64-79)."""
     os.makedirs(directory, exist_ok=True)
     write_plt(os.path.join(directory, "mesh.plt"), case.mesh)
     write_lobatto(os.path.join(directory, "velocity.txt"), case.eta, case.w)
+    with open(os.path.join(directory, "partition.txt"), "w") as fh:
+        fh.write("0\n" * case.mesh.npoin)
     p = case.prm
     keys = {
         "TORDER": int(case.torder), "rv": p["rv"], "FORCEOUT": int(forceout),
@@ -251,6 +275,7 @@ def write_run_dir(case, directory: str, ntime: int, forceout: int = 1,
         "INF": int(p.get("inf", 1)), "CINF": list(p["cinf"]), "NVSPACEPART": 1,
         "R": p["rgas"], "d": p["d"], "M": p["m"],
         "LobattoFile": "velocity.txt", "PSpaceFile": "mesh.plt", "OutFile": "FILE.OUT",
+        "PartitionFile": "partition.txt",
         "ResidualFile": "RESIDUAL.RES", "ResultsFile1": "RESULTS1.RES",
         "ResultsFile2": "RESULTS2.RES", "RestartOutFile": "RESTART_OUT.RES",
         "GIDMeshFile": "GIDMESH.RES",

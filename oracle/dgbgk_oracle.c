@@ -712,7 +712,10 @@ static int ordinate(ora_ctx *c, int iv, double dte)
     }
     double scpr = ux * c->nx[is - 1] + uy * c->ny[is - 1];
     double edgun[2];
-    if (scpr > 0.0) {
+    if (c->prm.mutant == ORA_MUTANT_SWAP_NODES) { int t = inl1; inl1 = inl2; inl2 = t; }
+    int upwind_left = scpr > 0.0;
+    if (c->prm.mutant == ORA_MUTANT_DOWNWIND) upwind_left = !upwind_left;
+    if (upwind_left) {
       edgun[0] = F(c, inl1, iv, iel) - c->umean[iel - 1];
       edgun[1] = F(c, inl2, iv, iel) - c->umean[iel - 1];
     } else {
@@ -741,6 +744,7 @@ static int ordinate(ora_ctx *c, int iv, double dte)
       double nf = F(c, in, iv, ie);
       double coll = c->mu[k] * (c->nfo[k] - nf);
       double r1 = c->mmat[k] * c->rhs[k] + coll * dte;
+      if (c->prm.mutant == ORA_MUTANT_RES_AFTER && c->inflow_node[k]) r1 = 0.0;
       if (fabs(r1) > fabs(res)) res = fabs(r1);
       c->delun[k] = c->delun[k] + c->mmat[k] * c->rhs[k] + coll * dte;
     }
@@ -809,7 +813,7 @@ int ora_step(ora_ctx *c, int nsteps, ora_step_info *info)
     for (int iv = 0; iv < c->V; iv++)
       if (fabs(c->residual[iv]) > sumres) sumres = fabs(c->residual[iv]);
     macros(c);
-    if (c->itime % 1000 == 0) getmac(c);
+    if (c->itime % 1000 == 0 || c->prm.mutant == ORA_MUTANT_GETMAC_EVERY) getmac(c);
     if (c->itime == 1) c->sritm1 = sumres;
     if (info) {
       info[s].sumres = sumres;

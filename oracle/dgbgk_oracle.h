@@ -40,7 +40,17 @@ typedef struct ora_params {
   int lookup;             /* 0 = reference ring lookup, 1 = corrected */
   int jacobi;             /* 0 = in-place velocity sweep (reference), 1 = Jacobi */
   int faithful_scans;     /* 1 = literal O(N^2) searches of the reference */
+  int mutant;             /* 0 = the restatement; otherwise a planted mistake,
+                             used only by the tests that show a pin catches it:
+                             ORA_MUTANT_* below */
 } ora_params;
+
+/* Planted mistakes (tests only).  Each is a plausible misreading of the
+ * reference; the pins must reject every one of them. */
+#define ORA_MUTANT_DOWNWIND 1     /* EDGFLXOPT takes the downwind element's values */
+#define ORA_MUTANT_SWAP_NODES 2   /* EDGFLXOPT reads ISIDE(5:6) swapped */
+#define ORA_MUTANT_GETMAC_EVERY 3 /* GETMAC's point gather every step, not every 1000th */
+#define ORA_MUTANT_RES_AFTER 4    /* residual taken after INFLOW's reset */
 
 typedef struct ora_ctx ora_ctx;
 

@@ -20,7 +20,8 @@ class OraParams(C.Structure):
                 ("rgas", C.c_double), ("d", C.c_double), ("m", C.c_double),
                 ("t1", C.c_double), ("p1", C.c_double), ("u0", C.c_double),
                 ("v0", C.c_double), ("cinf", C.c_double * 4), ("inf", C.c_int),
-                ("lookup", C.c_int), ("jacobi", C.c_int), ("faithful_scans", C.c_int)]
+                ("lookup", C.c_int), ("jacobi", C.c_int), ("faithful_scans", C.c_int),
+                ("mutant", C.c_int)]
 
 
 class OraStepInfo(C.Structure):
@@ -83,7 +84,11 @@ def lib():
     return _lib
 
 
-def make_params(prm: dict, lookup: int = 1, jacobi: int = 0, faithful: int = 0) -> OraParams:
+MUTANT_DOWNWIND, MUTANT_SWAP_NODES, MUTANT_GETMAC_EVERY, MUTANT_RES_AFTER = 1, 2, 3, 4
+
+
+def make_params(prm: dict, lookup: int = 1, jacobi: int = 0, faithful: int = 0,
+                mutant: int = 0) -> OraParams:
     p = OraParams()
     for k in ("rv", "csafm", "alpha", "rgas", "d", "m", "t1", "p1", "u0", "v0"):
         setattr(p, k, float(prm[k]))
@@ -91,13 +96,15 @@ def make_params(prm: dict, lookup: int = 1, jacobi: int = 0, faithful: int = 0) 
         p.cinf[i] = float(prm["cinf"][i])
     p.inf = int(prm.get("inf", 1))
     p.lookup, p.jacobi, p.faithful_scans = int(lookup), int(jacobi), int(faithful)
+    p.mutant = int(mutant)
     return p
 
 
 class OracleRun:
     """One oracle context on a Case (see dgbgk.cases)."""
 
-    def __init__(self, case, lookup: int = 1, jacobi: int = 0, faithful: int = 0):
+    def __init__(self, case, lookup: int = 1, jacobi: int = 0, faithful: int = 0,
+                 mutant: int = 0):
         L = lib()
         m = case.mesh
         self.case = case
@@ -106,7 +113,7 @@ class OracleRun:
         self._intma = np.ascontiguousarray(m.intma, dtype=np.int32)
         self._coord = np.ascontiguousarray(m.coord, dtype=np.float64)
         self._bsido = np.ascontiguousarray(m.bsido, dtype=np.int32)
-        self._prm = make_params(case.prm, lookup, jacobi, faithful)
+        self._prm = make_params(case.prm, lookup, jacobi, faithful, mutant)
         h = C.c_void_p()
         st = L.ora_create(self.E, self.P, self.B, self._intma, self._coord, self._bsido,
                           case.rorder, case.torder, np.ascontiguousarray(case.eta, float),

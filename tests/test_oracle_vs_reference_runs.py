@@ -1,0 +1,112 @@
+"""The oracle against the reference program's own runs (golden fixtures).
+
+tests/golden/ref_runs/<deck>/ holds what the reference's whole program,
+built from its unmodified sources (default REAL(4)) and run at one MPI
+rank, wrote for two decks over 1010 steps (tests/golden/make_reference_runs.py):
+every step's LOG(SUMRES/SRITM1), lift and drag (RESIDUAL.RES), and the
+point fields at steps 0 and 1000 (RESULTS1/2.RES).  The FP64 oracle, run
+on the same inputs rounded to REAL(4) as the reference reads them, must
+agree to the reference's own single-precision level.  This pins, from the
+reference's code, what the oracle's closed forms cannot: ADVNCE's
+composition and order (FLUCON2, COMPMU, ALOTIM2, the in-place ordinate
+sweep, GETRES, MACROS, GETMAC every 1000th step, SRITM1), EDGFLXOPT's
+interior-side loop, and the stale-pressure forces.
+
+Tolerances (measured agreement in brackets): log residual 1e-5 absolute
+[1.2e-6, 1.5e-6]; forces 2e-4 of |lift| + |drag| plus 1e-6 of the wall
+pressure integrand, the single-precision floor of a cancelling sum
+[step 1000: 2.4e-6 of 0.023, 2.2e-5 of 0.343]; point fields 2e-4 relative
+L2, velocity as a vector [1.6e-5 .. 5.3e-5].
+The second test plants four plausible misreadings in the oracle (the
+downwind element's values on interior sides, ISIDE(5:6) read swapped,
+GETMAC's gather every step, the residual after INFLOW's reset) and checks
+that each breaks the agreement.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from dgbgk import cases, io
+from small_cases import GOLDEN, rel_l2, rel_l2_vec
+
+RUNS = os.path.join(GOLDEN, "ref_runs")
+DECKS = ("lobatto10_walls", "cylinder_lobatto10")
+TOL_LOGRES, TOL_FORCE, TOL_FIELD = 1e-5, 2e-4, 2e-4
+
+
+def _golden(name):
+    d = os.path.join(RUNS, name)
+    r1 = io.read_results(os.path.join(d, "RESULTS1.RES"))
+    r2 = io.read_results(os.path.join(d, "RESULTS2.RES"))
+    return (cases.from_run_dir(d, single=True), io.read_residual(os.path.join(d, "RESIDUAL.RES")),
+            r1, r2)
+
+
+def _fields_err(pf, r1, r2):
+    """Relative L2 of the oracle's point fields against one RESULTS block."""
+    return {"nd": rel_l2(pf["nd"], r1[:, 0]),
+            "uv": rel_l2_vec(pf["uvel"], pf["vvel"], r1[:, 1], r1[:, 2]),
+            "rho": rel_l2(pf["rho"], r2[:, 0]), "ps": rel_l2(pf["ps"], r2[:, 1]),
+            "temp": rel_l2(pf["temp"], r2[:, 2])}
+
+
+def _compare(case, res, r1, r2, mutant=0, nsteps=1000):
+    """Runs the oracle nsteps on the golden deck; returns the largest errors
+    (nan where a run blew up)."""
+    from oracle_py import OracleRun
+    o = OracleRun(case, lookup=0, mutant=mutant)   # the reference's ring rule
+    err0 = _fields_err(o.point_fields(), r1[0], r2[0])
+    recs = o.step(nsteps)
+    logres = np.array([r["logres"] for r in recs])
+    lift = np.array([r["lift"] for r in recs])
+    drag = np.array([r["drag"] for r in recs])
+    # forces: relative to |lift| + |drag|, with a single-precision floor of
+    # 1e-6 of the pressure integrand's size (wall length x pressure), which
+    # the cancelling sums of a uniform state round to
+    m = case.mesh
+    wall = m.bsido[:, 3] == 4
+    p0, p1 = m.coord[m.bsido[wall, 0] - 1], m.coord[m.bsido[wall, 1] - 1]
+    f0 = float(np.sum(np.hypot(*(p1 - p0).T)) * np.max(r2[0][:, 1]))
+    allowed = TOL_FORCE * (np.abs(res[:nsteps, 2]) + np.abs(res[:nsteps, 3])) + 1e-6 * f0
+    out = {"logres": float(np.max(np.abs(logres - res[:nsteps, 1]))),
+           "force": float(TOL_FORCE * np.max((np.abs(lift - res[:nsteps, 2]) +
+                                              np.abs(drag - res[:nsteps, 3])) / allowed)),
+           "fields0": max(err0.values()),
+           "fields1000": max(_fields_err(o.point_fields(), r1[1000], r2[1000]).values())}
+    o.close()
+    return {k: (v if np.isfinite(v) else np.inf) for k, v in out.items()}
+
+
+@pytest.mark.parametrize("name", DECKS)
+def test_oracle_matches_reference_program(name):
+    case, res, r1, r2 = _golden(name)
+    assert res.shape == (1010, 4) and sorted(r1) == [0, 1000, 1011]
+    assert case.rorder == case.torder == 10 and case.mesh.nelem <= 30000
+    e = _compare(case, res, r1, r2)
+    assert e["logres"] < TOL_LOGRES, e
+    assert e["force"] < TOL_FORCE, e
+    assert e["fields0"] < TOL_FIELD, e
+    assert e["fields1000"] < TOL_FIELD, e
+
+
+@pytest.mark.parametrize("mutant", [1, 2, 3, 4],
+                         ids=["downwind", "swapped_nodes", "getmac_every_step",
+                              "residual_after_inflow"])
+def test_golden_runs_reject_planted_mistakes(mutant):
+    """Each planted misreading fails at least one of the golden checks by a
+    wide margin (at least 10x its tolerance)."""
+    case, res, r1, r2 = _golden("lobatto10_walls")
+    e = _compare(case, res, r1, r2, mutant=mutant)
+    worst = max(e["logres"] / TOL_LOGRES, e["force"] / TOL_FORCE,
+                e["fields1000"] / TOL_FIELD)
+    assert worst > 10.0, e
+
+
+def test_reference_timing_recorded():
+    """The reference's own one-rank wall time is kept beside each golden run
+    (a CPU baseline of the reference itself, not a parity check)."""
+    import json
+    for name in DECKS:
+        t = json.load(open(os.path.join(RUNS, name, "timing.json")))
+        assert t["steps"] == 1010 and t["mpi_ranks"] == 1 and t["wall_s"] > 0
