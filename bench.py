@@ -1,14 +1,16 @@
 """Benchmark: DOF-velocity updates per second of the DG-BGK time march.
 
-Workload (default): the plain FP64 cylinder flow -- 20D x 10D box with a
-cylinder of diameter D, ~100k-triangle graded Delaunay mesh, 16 x 32 = 512
-polar ordinates, Mach 0.5, Kn 0.05, diffuse isothermal wall, the
-reference's in-place velocity sweep.  A step is one full ADVNCE loop body
-(wall eta, update of every (node, ordinate), wall pass, moments and
-collision frequency, residual, forces, next time step); f is resident in
-HBM before the timed region starts.
+Workload (default): the particle-drag case -- a circular particle of
+diameter D in a 200D x 100D box, ~1M-triangle graded Delaunay mesh
+(1,005,669 triangles), 32 x 64 = 2048 polar ordinates, Mach 0.3, Kn 1,
+diffuse isothermal wall, the reference's in-place velocity sweep, plain
+FP64.  A step is one full ADVNCE loop body (wall eta, update of every
+(node, ordinate), wall pass, moments and collision frequency, residual,
+forces, next time step); f is resident in HBM before the timed region
+starts.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32]
+                  [--config particle|cylinder]
 
 Prints one JSON line.  value = 3*NELEM*VNPNT*K / wall seconds of the K
 timed steps, run as the library runs them (replayed HIP graphs).
@@ -16,10 +18,16 @@ roofline: the update kernel (k_update) against HBM, with algorithmic bytes
 = one read + one write of f per DOF-velocity update, divided by its average
 launch time over K more steps launched with HIP events around every kernel
 on the library's stream.
-cpu_baseline: the FP64 C oracle (one core) on a bounded sample of the same
-workload (2 steps).
-time_to_residual (cylinder configuration; --no-ttr skips it): steps and wall
-time to a step change of 1e-6 of the plain run's step-1 change on the same
+Extra keys (default run): "fp32_bulk", the same workload and measurement
+in the FP32-bulk form; "cylinder", the ~100k x 512 cylinder (Mach 0.5,
+Kn 0.05) in both forms.
+cpu_baseline: the FP64 C oracle, threaded over all host threads of the
+run (the same values bit for bit as its serial order), one step of a
+bounded sample of the workload: the same mesh and flow with 2 x 64
+ordinates (DOF-velocity updates per second are per ordinate and element,
+so the rate carries over).
+time_to_residual (--no-ttr skips it): steps and wall time to a step change
+of 1e-6 of the plain run's step-1 change on the ~100k-triangle cylinder
 geometry held at the freestream on all box sides, FP64 from the freestream
 against FP32-bulk from the committed warm-start network's state (about a
 minute).
@@ -27,6 +35,7 @@ minute).
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -66,16 +75,13 @@ def copy_calibration(nbytes: int):
 
 
 def time_to_residual():
-    """The metric's second half on the headline geometry: steps and wall time
+    """The metric's second half on the cylinder geometry: steps and wall time
     to a step change of 1e-6 of the plain run's step-1 change, FP64 from the
     freestream (plain) against FP32-bulk from the committed network's state
     (learned), on the 101,753-triangle cylinder (16 x 32 ordinates, freestream
     held on all box sides, as steady runs need).  One held-out case near the
     median of the eight evaluated: Mach 0.82, Kn 0.17."""
     from dgbgk import _lib, meshgen, warmstart
-
-    def log(msg):
-        print(msg, file=sys.stderr, flush=True)
     params = os.path.join(ROOT, "dg-bgk_amd", "dgbgk", "data", "warmstart_cylinder.npz")
     mesh = warmstart.mesh_for(100_000)
     smp = warmstart.Sample(0.82, 0.17)
@@ -105,15 +111,152 @@ def time_to_residual():
             "drag_rel": abs(learned["drag"] - plain["drag"]) / abs(plain["drag"])}
 
 
+KERNELS = {0: "update", 1: "wall", 2: "moments", 3: "eta", 4: "final", 5: "points"}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_case(config: str, elems: int = 0, mesh: str = "", shuffle: int = -1):
+    """(case, workload text, mach, kn) of a bench workload."""
+    from dgbgk import cases, io, meshgen
+    if mesh:
+        case = cases.cylinder_flow(target_elems=2000)
+        case.mesh = io.read_plt(mesh)
+        wl = (f"mesh {os.path.basename(mesh)} x 16x32 ordinates, Mach 0.5, Kn 0.05 "
+              "(cylinder flow conditions)")
+        mach, kn = 0.5, 0.05
+    elif config == "cylinder":
+        case = cases.cylinder_flow(target_elems=elems or 100_000)
+        wl = "cylinder flow, 20Dx10D box, ~100k triangles x 16x32 ordinates, Mach 0.5, Kn 0.05"
+        mach, kn = 0.5, 0.05
+    else:
+        case = cases.particle_drag(target_elems=elems or 1_000_000)
+        wl = ("particle drag, 200Dx100D box, ~1M triangles x 32x64 ordinates, "
+              "Mach 0.3, Kn 1")
+        mach, kn = 0.3, 1.0
+    if shuffle >= 0:
+        case.mesh, _ = meshgen.shuffle_elements(case.mesh, shuffle)
+        wl += f", elements in random order (seed {shuffle})"
+    return case, wl, mach, kn
+
+
+def measure(case, precision: str, steps: int, warmup: int, vchunk: int = 0,
+            traffic_key: str | None = None) -> dict:
+    """Times `steps` steps as the library runs them (replayed HIP graphs),
+    then the same number launched one kernel at a time with HIP events
+    around every launch; frees the device context before returning."""
+    from dgbgk import _lib
+    from dgbgk.solver import Solver
+    prec = _lib.DGBGK_FP64 if precision == "fp64" else _lib.DGBGK_FP32
+    t0 = time.time()
+    s = Solver(case, precision=prec, vchunk=vchunk)
+    E, V = case.mesh.nelem, case.nvel
+    dofv = 3 * E * V
+    log(f"# {case.name} E={E} V={V} ({case.rorder}x{case.torder}) {precision} "
+        f"setup {time.time() - t0:.1f}s")
+    log("# launch " + " ".join(f"{k}={v}" for k, v in s.launch_config().items()))
+    s.step(warmup, records=False)
+    s.sync()
+    # timed region: K steps with no host synchronisation between them
+    t1 = time.perf_counter()
+    recs = s.step(steps)
+    s.sync()
+    wall = time.perf_counter() - t1
+    # kernel timing: the same K steps, HIP events around every launch
+    s.timing(True)
+    t3 = time.perf_counter()
+    s.step(steps, records=False)
+    s.sync()
+    ev_wall = time.perf_counter() - t3
+    kern = {KERNELS[k]: s.kernel_time(k) for k in KERNELS}
+    s.timing(False)
+    log("# kernel ms/launch: " + ", ".join(
+        f"{n}={ms / max(c, 1):.4f}" for n, (ms, c) in kern.items() if c))
+    upd_ms, upd_n = kern["update"]
+    bpv = s.bytes_per_value()
+    lc = s.launch_config()
+    s.close()
+    avg_upd_s = (upd_ms / max(upd_n, 1)) / 1e3
+    alg_bytes = 2.0 * bpv * dofv  # one read + one write of f per update
+    achieved = alg_bytes / avg_upd_s / 1e9 if upd_n else None
+    traffic = _traffic_from_profiles(traffic_key, precision) if traffic_key else None
+    return {
+        "value": dofv * steps / wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": traffic,
+                     "kernel": "k_update", "alg_bytes_per_launch": alg_bytes,
+                     "avg_launch_ms": avg_upd_s * 1e3},
+        "kernels_ms_per_launch": {n: ms / c for n, (ms, c) in kern.items() if c},
+        "ms_per_step_event_timed": 1e3 * ev_wall / steps,
+        "step_bytes_alg_per_s": 3.0 * bpv * dofv * steps / wall,
+        "launch": lc,
+        "residual": {"first": recs[0]["sumres"], "last": recs[-1]["sumres"],
+                     "logres_last": recs[-1]["logres"]},
+        "mass": {"first": recs[0]["mass"], "last": recs[-1]["mass"]},
+    }
+
+
+def host_threads() -> int:
+    """Threads for the CPU baseline: the run's OMP_NUM_THREADS (the GPU box
+    sets 16, its CPU share), else the visible CPUs."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline(config: str, elems: int, serial: bool) -> dict:
+    """The FP64 C oracle on a bounded sample of the workload: one step of the
+    same mesh and flow with 2 speed rings x 64 angles (Gauss-Legendre radial
+    nodes), threaded over the host threads (bit-identical to one thread);
+    optionally also on one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from dgbgk import cases
+    from oracle_py import OracleRun
+    if config == "cylinder":
+        case = cases.cylinder_flow(target_elems=elems or 100_000, rorder=2, torder=64)
+    else:
+        case = cases.particle_drag(target_elems=elems or 1_000_000, rorder=2, torder=64)
+    o = OracleRun(case)
+    dofv = 3 * case.mesh.nelem * case.nvel
+    nt = host_threads()
+    o.set_threads(nt)
+    t0 = time.perf_counter()
+    o.step(1)
+    dt = time.perf_counter() - t0
+    cpu = None
+    for line in open("/proc/cpuinfo"):
+        if line.startswith("model name"):
+            cpu = line.split(":", 1)[1].strip()
+            break
+    out = {"value": dofv / dt, "unit": "DOF-velocity updates/s", "cores": nt, "kind": "port",
+           "sample": f"1 step of the {config} mesh (E={case.mesh.nelem}) and flow with 2 x 64 "
+                     f"ordinates ({dofv:.3e} DOF-velocity updates), FP64 C oracle in the "
+                     f"reference's order, {nt} threads, {dt:.1f} s",
+           "host": {"nproc": os.cpu_count(), "cpu": cpu or platform.processor()}}
+    if serial:
+        o.set_threads(1)
+        t0 = time.perf_counter()
+        o.step(1)
+        ds = time.perf_counter() - t0
+        out["serial"] = {"value": dofv / ds, "cores": 1, "seconds": ds}
+    o.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--config", choices=["cylinder", "particle"], default="cylinder",
-                    help="cylinder: 100k triangles x 512 ordinates (headline); "
-                         "particle: 1M triangles x 2048 ordinates")
+    ap.add_argument("--config", choices=["particle", "cylinder"], default="particle",
+                    help="particle: 1M triangles x 2048 ordinates (headline); "
+                         "cylinder: 100k triangles x 512 ordinates")
     ap.add_argument("--elems", type=int, default=0, help="override the mesh size")
     ap.add_argument("--mesh", default="",
                     help="a mesh file in the reference's .plt format, run with the "
@@ -122,11 +265,14 @@ def main():
                     help="renumber the mesh's elements at random (an input mesh in "
                          "arbitrary element order)")
     ap.add_argument("--vchunk", type=int, default=0, help="ordinates per update block (0 = auto)")
+    ap.add_argument("--only", action="store_true",
+                    help="measure the named configuration and precision only (no extra keys)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-serial", action="store_true",
+                    help="also time the CPU baseline sample on one thread")
     ap.add_argument("--no-ttr", action="store_true",
                     help="skip the time-to-residual comparison (learned FP32-bulk vs plain "
-                         "FP64, about a minute; run with the cylinder configuration only)")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+                         "FP64 on the cylinder geometry, about a minute)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run three device-to-device copies of BYTES bytes (the "
                          "reference kernel that scales rocprofv3's FETCH_SIZE/WRITE_SIZE)")
@@ -140,90 +286,38 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from dgbgk import _lib, cases, io, meshgen
-    from dgbgk.solver import Solver
-
-    t0 = time.time()
-    if args.mesh:
-        case = cases.cylinder_flow(target_elems=2000)
-        case.mesh = io.read_plt(args.mesh)
-        wl = (f"mesh {os.path.basename(args.mesh)} x 16x32 ordinates, Mach 0.5, Kn 0.05 "
-              "(cylinder flow conditions)")
-        mach, kn = 0.5, 0.05
-    elif args.config == "cylinder":
-        case = cases.cylinder_flow(target_elems=args.elems or 100_000)
-        wl = "cylinder flow, 20Dx10D box, ~100k triangles x 16x32 ordinates, Mach 0.5, Kn 0.05"
-        mach, kn = 0.5, 0.05
-    else:
-        case = cases.particle_drag(target_elems=args.elems or 1_000_000)
-        wl = "particle drag, ~1M triangles x 32x64 ordinates, Mach 0.3, Kn 1"
-        mach, kn = 0.3, 1.0
-    if args.shuffle >= 0:
-        case.mesh, _ = meshgen.shuffle_elements(case.mesh, args.shuffle)
-        wl += f", elements in random order (seed {args.shuffle})"
-    prec = _lib.DGBGK_FP64 if args.precision == "fp64" else _lib.DGBGK_FP32
-    s = Solver(case, precision=prec, vchunk=args.vchunk)
-    setup_s = time.time() - t0
+    case, wl, mach, kn = make_case(args.config, args.elems, args.mesh, args.shuffle)
     E, V = case.mesh.nelem, case.nvel
-    dofv = 3 * E * V
-    print(f"# workload {args.config} E={E} V={V} ({case.rorder}x{case.torder}) "
-          f"{args.precision} setup {setup_s:.1f}s", file=sys.stderr, flush=True)
-    print("# launch " + " ".join(f"{k}={v}" for k, v in s.launch_config().items()),
-          file=sys.stderr, flush=True)
+    plain = not args.mesh and args.shuffle < 0
+    key = f"{args.config}:{E}x{V}" if plain else None
+    main_m = measure(case, args.precision, args.steps, args.warmup, args.vchunk, key)
 
-    s.step(args.warmup, records=False)
-    s.sync()
-    # timed region: K steps as the library runs them (captured HIP graphs
-    # of 64 steps, no host synchronisation between steps)
-    t1 = time.perf_counter()
-    recs = s.step(args.steps)
-    s.sync()
-    t2 = time.perf_counter()
-    wall = t2 - t1
-    # kernel timing: the same K steps launched one kernel at a time with
-    # HIP events around every launch on the library's stream
-    s.timing(True)
-    t3 = time.perf_counter()
-    s.step(args.steps, records=False)
-    s.sync()
-    t4 = time.perf_counter()
-    upd_ms, upd_n = s.kernel_time(0)
-    names = {0: "update", 1: "wall", 2: "moments", 3: "eta", 4: "final", 5: "points"}
-    kern = {names[k]: s.kernel_time(k) for k in names}
-    s.timing(False)
-    print("# kernel ms/launch: " + ", ".join(
-        f"{n}={ms / max(c, 1):.4f}" for n, (ms, c) in kern.items() if c), file=sys.stderr)
-
-    bpv = s.bytes_per_value()
-    value = dofv * args.steps / wall
-    avg_upd_s = (upd_ms / max(upd_n, 1)) / 1e3
-    alg_bytes = 2.0 * bpv * dofv  # one read + one write of f per update
-    achieved = alg_bytes / avg_upd_s / 1e9 if upd_n else None
-    traffic = (_traffic_from_profiles(f"{args.config}:{E}x{V}", args.precision)
-               if not args.mesh and args.shuffle < 0 else None)
+    extra = {}
+    if not args.only and plain and not args.elems:
+        other = "fp32" if args.precision == "fp64" else "fp64"
+        extra["fp32_bulk" if other == "fp32" else "fp64_plain"] = measure(
+            case, other, args.steps, args.warmup, args.vchunk, key)
+        if args.config == "particle":
+            cyl, cwl, _, _ = make_case("cylinder")
+            ck = f"cylinder:{cyl.mesh.nelem}x{cyl.nvel}"
+            extra["cylinder"] = {"workload": cwl, "nelem": cyl.mesh.nelem,
+                                 "ordinates": cyl.nvel,
+                                 "fp64": measure(cyl, "fp64", 100, 10, 0, ck),
+                                 "fp32_bulk": measure(cyl, "fp32", 100, 10, 0, ck)}
+            del cyl
 
     cpu = None
     if not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        from oracle_py import OracleRun
-        o = OracleRun(case)
-        tc0 = time.perf_counter()
-        o.step(args.cpu_steps)
-        tc = time.perf_counter() - tc0
-        cpu = {"value": dofv * args.cpu_steps / tc, "unit": "DOF-velocity updates/s",
-               "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same {args.config} workload "
-                         f"(E={E}, V={V}), FP64 C oracle, reference order, {tc:.1f} s"}
-        o.close()
+        cpu = cpu_baseline(args.config, args.elems, args.cpu_serial)
 
     out = {
         "metric": "DOF-velocity updates/s + time-to-residual 1e-6, learned vs plain, 1 MI355X",
-        "value": value,
+        "value": main_m["value"],
         "unit": "DOF-velocity updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "ms_per_step": main_m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -233,19 +327,16 @@ def main():
                    "nelem": E, "npoin": case.mesh.npoin, "ordinates": V,
                    "rorder": case.rorder, "torder": case.torder, "mach": mach, "kn": kn,
                    "order": "in-place velocity sweep", "precision": args.precision},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": traffic,
-                     "kernel": "k_update", "alg_bytes_per_launch": alg_bytes,
-                     "avg_launch_ms": avg_upd_s * 1e3},
-        "kernels_ms_per_launch": {n: ms / c for n, (ms, c) in kern.items() if c},
-        "ms_per_step_event_timed": 1e3 * (t4 - t3) / args.steps,
-        "residual": {"first": recs[0]["sumres"], "last": recs[-1]["sumres"],
-                     "logres_last": recs[-1]["logres"]},
+        "roofline": main_m["roofline"],
+        "kernels_ms_per_launch": main_m["kernels_ms_per_launch"],
+        "ms_per_step_event_timed": main_m["ms_per_step_event_timed"],
+        "launch": main_m["launch"],
+        "residual": main_m["residual"],
+        "mass": main_m["mass"],
         "cpu_baseline": cpu,
     }
-    if (args.config == "cylinder" and not args.no_ttr and not args.elems and not args.mesh
-            and args.shuffle < 0):
+    out.update(extra)
+    if not args.no_ttr and not args.only and plain and not args.elems:
         out["time_to_residual"] = time_to_residual()
     print(json.dumps(out))
 

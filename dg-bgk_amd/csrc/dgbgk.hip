@@ -1067,6 +1067,37 @@ __global__ void __launch_bounds__(256) k_seed(T* __restrict__ f, const int* __re
   f[(size_t)v * ldf + k] = (T)(n1 * f0);
 }
 
+// Minimum, maximum and count of non-finite values of f, per block: rows v
+// of blockIdx.y, grid-stride over the 3E node values of the row (the row
+// padding is not read).  The host combines the blocks' partials.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fstats(const T* __restrict__ f, int E, long long ldf,
+                                                double* __restrict__ part) {
+  __shared__ double sh[8];
+  const size_t N3 = 3 * (size_t)E;
+  const T* row = f + (size_t)blockIdx.y * ldf;
+  double lo = INFINITY, hi = -INFINITY, bad = 0.0;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < N3;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const double x = (double)row[k];
+    if (isfinite(x)) {
+      lo = fmin(lo, x);
+      hi = fmax(hi, x);
+    } else {
+      bad += 1.0;
+    }
+  }
+  const double mlo = -block_max(-lo, sh);
+  const double mhi = block_max(hi, sh);
+  const double nb = block_sum(bad, sh);
+  if (threadIdx.x == 0) {
+    double* o = part + 3 * ((size_t)blockIdx.y * gridDim.x + blockIdx.x);
+    o[0] = mlo;
+    o[1] = mhi;
+    o[2] = nb;
+  }
+}
+
 // Layout conversions between the reference's DISNF(3,V,E) (double, input
 // element numbering) and the device's velocity-major T[V][ldf] (storage
 // order; eorder maps a storage position to its input element, or is null
@@ -2165,6 +2196,39 @@ int dgbgk_kernel_time(dgbgk_ctx* h, int kernel, double* ms_total, long* count) {
 int dgbgk_sync(dgbgk_ctx* h) {
   if (!h) return fail(kErrArg, "null context");
   HIPCK(hipStreamSynchronize(h->c.stream));
+  return kOk;
+}
+
+int dgbgk_f_stats(dgbgk_ctx* h, double* fmin, double* fmax, long long* nonfinite) {
+  if (!h) return fail(kErrArg, "null context");
+  Ctx& c = h->c;
+  const size_t N3 = 3 * (size_t)c.p.E;
+  const unsigned gx = (unsigned)std::min<size_t>(16, (N3 + 255) / 256);
+  const size_t nb = (size_t)gx * c.p.V;
+  double* d = nullptr;
+  HIPCK(hipMalloc((void**)&d, 3 * nb * sizeof(double)));
+  dim3 g(gx, c.p.V);
+  if (c.prec)
+    hipLaunchKernelGGL(k_fstats<float>, g, dim3(256), 0, c.stream, (const float*)c.f[c.cur],
+                       c.p.E, (long long)c.ldf, d);
+  else
+    hipLaunchKernelGGL(k_fstats<double>, g, dim3(256), 0, c.stream, (const double*)c.f[c.cur],
+                       c.p.E, (long long)c.ldf, d);
+  std::vector<double> part(3 * nb);
+  hipError_t e1 = hipMemcpyAsync(part.data(), d, 3 * nb * sizeof(double), hipMemcpyDeviceToHost,
+                                 c.stream);
+  hipError_t e2 = hipStreamSynchronize(c.stream);
+  hipFree(d);
+  if (e1 != hipSuccess || e2 != hipSuccess) return fail(kErrHip, "f_stats failed");
+  double lo = INFINITY, hi = -INFINITY, bad = 0.0;
+  for (size_t b = 0; b < nb; ++b) {
+    lo = std::min(lo, part[3 * b]);
+    hi = std::max(hi, part[3 * b + 1]);
+    bad += part[3 * b + 2];
+  }
+  if (fmin) *fmin = lo;
+  if (fmax) *fmax = hi;
+  if (nonfinite) *nonfinite = (long long)bad;
   return kOk;
 }
 

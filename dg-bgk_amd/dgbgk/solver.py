@@ -186,6 +186,13 @@ class Solver:
     def sync(self):
         check(lib().dgbgk_sync(self.h))
 
+    def f_stats(self) -> dict:
+        """min and max of the finite values of f and the count of non-finite
+        ones, computed on the device over every node and ordinate."""
+        lo, hi, bad = C.c_double(), C.c_double(), C.c_longlong()
+        check(lib().dgbgk_f_stats(self.h, C.byref(lo), C.byref(hi), C.byref(bad)))
+        return {"min": lo.value, "max": hi.value, "nonfinite": bad.value}
+
     def bytes_per_value(self) -> int:
         b = C.c_int()
         check(lib().dgbgk_dof_bytes(self.h, C.byref(b)))

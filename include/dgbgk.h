@@ -125,6 +125,12 @@ int dgbgk_forces(dgbgk_ctx *ctx, double *lift, double *drag);
 /* FLUCON2's ETA per boundary side (src/flucon2.f90:218-241), last step. */
 int dgbgk_wall_eta(dgbgk_ctx *ctx, double *eta);
 
+/* Smallest and largest finite value of the current f over every node and
+ * ordinate, and the count of non-finite values (no counterpart in the
+ * reference: a device-side check of f >= 0 and finiteness that needs no
+ * host copy of f). */
+int dgbgk_f_stats(dgbgk_ctx *ctx, double *fmin, double *fmax, long long *nonfinite);
+
 /* Sizes: NSIDE, VNPNT, NELEM, NPOIN, NBOUN, steps taken. */
 int dgbgk_sizes(dgbgk_ctx *ctx, int *nside, int *nvel, int *nelem, int *npoin,
                 int *nboun, int *itime);

@@ -264,6 +264,11 @@ struct ora_ctx {
   unsigned char *inflow_node; /* 1 if DG node is reset by INFLOW */
   int itime;
   double sritm1;
+  /* threaded stepping (ora_set_threads): each element's sides in increasing
+   * side number, +is for the left element, -is for the right one */
+  int nthreads;
+  int *elem_sides;
+  double *node_si, *node_su, *node_sv, *node_sp;
 };
 
 static void *zal(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
@@ -278,10 +283,11 @@ double ora_sumweight(const ora_ctx *c) { return c->sumweight; }
 #define FS(c, in, iv, ie) (c)->fsnap[(((size_t)(ie)-1) * (c)->V + ((iv)-1)) * 3 + ((in)-1)]
 
 /* GTEQNF: local Maxwellian for one ordinate (src/gteqnf.f90:414-432). */
-void ora_gteqnf(int nelem, const double *disnd, const double *disux,
-                const double *disuy, const double *disps, double ux, double uy,
-                double m, double *nfo)
+static void gteqnf_n(int nthreads, int nelem, const double *disnd, const double *disux,
+                     const double *disuy, const double *disps, double ux, double uy,
+                     double m, double *nfo)
 {
+#pragma omp parallel for schedule(static) if (nthreads > 1) num_threads(nthreads)
   for (int ie = 1; ie <= nelem; ie++)
     for (int in = 1; in <= 3; in++) {
       size_t k = NE(in, ie);
@@ -296,6 +302,13 @@ void ora_gteqnf(int nelem, const double *disnd, const double *disux,
       double tmp = -beta2 * ((cx - ux) * (cx - ux) + (cy - uy) * (cy - uy));
       nfo[k] = nd * (beta2 / PI_R) * exp(tmp);
     }
+}
+
+void ora_gteqnf(int nelem, const double *disnd, const double *disux,
+                const double *disuy, const double *disps, double ux, double uy,
+                double m, double *nfo)
+{
+  gteqnf_n(1, nelem, disnd, disux, disuy, disps, ux, uy, m, nfo);
 }
 
 /* Inflow Maxwellian f_inf(c) (src/inflow.f90:150-161). */
@@ -338,6 +351,60 @@ void ora_getfor(int nboun, const int *bsido, const double *ps,
   *drag = d;
 }
 
+/* MACROS with the per-node velocity sums split over threads (each node's
+ * sums keep the ordinate order) and the point accumulation kept serial in
+ * element order, so every value equals the serial loop's bit for bit.  Used
+ * by the threaded CPU baseline only. */
+static void macros_threaded(ora_ctx *c, double frac, double mm)
+{
+  const int E = c->E, V = c->V;
+  const int *intma = c->intma;
+  const double *geome = c->geome, *vcord = c->vcord;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+  for (int ie = 1; ie <= E; ie++)
+    for (int in = 1; in <= 3; in++) {
+      size_t k = NE(in, ie);
+      c->disnd[k] = c->disux[k] = c->disuy[k] = c->disps[k] = 0.0;
+      double si = 0.0, su = 0.0, sv = 0.0, sp = 0.0;
+      for (int iv = 1; iv <= V; iv++) {
+        double w = VCORD(3, iv), jac = c->jac[iv - 1];
+        double fv = F(c, in, iv, ie);
+        si = si + fv * w * jac * frac;
+        su = su + c->cxb[iv - 1] * fv * w * jac * frac;
+        sv = sv + c->cyb[iv - 1] * fv * w * jac * frac;
+      }
+      if (si >= 1e-20) {
+        c->disnd[k] = si;
+        c->disux[k] = su / si;
+        c->disuy[k] = sv / si;
+      }
+      for (int iv = 1; iv <= V; iv++) {
+        double w = VCORD(3, iv), jac = c->jac[iv - 1];
+        double cxd = c->cxb[iv - 1] - c->disux[k];
+        double cyd = c->cyb[iv - 1] - c->disuy[k];
+        double tmp = 0.5 * (cxd * cxd + cyd * cyd);
+        sp = sp + tmp * F(c, in, iv, ie) * w * jac * frac;
+      }
+      c->node_si[k] = si; c->node_su[k] = su; c->node_sv[k] = sv; c->node_sp[k] = sp;
+      c->disps[k] = sp * mm;
+    }
+  for (int ie = 1; ie <= E; ie++) {
+    double ae = GEOME(7, ie);
+    for (int in = 1; in <= 3; in++) {
+      size_t k = NE(in, ie);
+      int ip = INTMA(in, ie);
+      double si = c->node_si[k];
+      if (si >= 1e-20) {
+        c->uvel_pp[ip - 1] = c->uvel_pp[ip - 1] + ae * c->node_su[k] / si;
+        c->vvel_pp[ip - 1] = c->vvel_pp[ip - 1] + ae * c->node_sv[k] / si;
+      }
+      c->flag[ip - 1] = c->flag[ip - 1] + ae;
+      c->nd_pp[ip - 1] = c->nd_pp[ip - 1] + si * ae;
+      c->ps_pp[ip - 1] = c->ps_pp[ip - 1] + mm * c->node_sp[k] * ae;
+    }
+  }
+}
+
 /* MACROS: node moments and area-weighted point averages (src/macros.f90:55-188). */
 static void macros(ora_ctx *c)
 {
@@ -349,6 +416,10 @@ static void macros(ora_ctx *c)
   for (int ip = 0; ip < P; ip++) {
     c->rho_pp[ip] = c->uvel_pp[ip] = c->vvel_pp[ip] = 0.0;
     c->temp_pp[ip] = c->ps_pp[ip] = c->nd_pp[ip] = c->flag[ip] = 0.0;
+  }
+  if (c->nthreads > 1) {
+    macros_threaded(c, frac, mm);
+    goto points;
   }
   for (int ie = 1; ie <= E; ie++) {
     double ae = GEOME(7, ie);
@@ -385,6 +456,7 @@ static void macros(ora_ctx *c)
       c->disps[k] = sp * mm;
     }
   }
+points:
   for (int ip = 0; ip < P; ip++) {
     if (c->nd_pp[ip] < 1e-20) {
       c->rho_pp[ip] = c->uvel_pp[ip] = c->vvel_pp[ip] = 0.0;
@@ -419,6 +491,7 @@ static void compmu(ora_ctx *c)
   const double *vcord = c->vcord;
   double frac = 4.0 / c->sumweight;
   double sigt = PI_R * c->prm.d * c->prm.d;
+#pragma omp parallel for schedule(static) if (c->nthreads > 1) num_threads(c->nthreads)
   for (int ie = 1; ie <= E; ie++)
     for (int in = 1; in <= 3; in++) {
       size_t k = NE(in, ie);
@@ -640,6 +713,91 @@ typedef struct ora_stage {
 } ora_stage;
 static ora_stage *g_stage = NULL;
 
+/* EDGFLXOPT's side loop, element by element for the threaded baseline: each
+ * element visits its own sides in increasing side number, computes each
+ * side's flux with the serial loop's arithmetic and adds it to its own two
+ * nodes, so every node receives the same terms in the same order as in the
+ * serial loop (an interior side's flux is computed by both its elements).
+ * Boundary sides call GETBOU as the serial loop does; it writes only that
+ * element's nodes. */
+static int edges_threaded(ora_ctx *c, int iv, double ux, double uy)
+{
+  const int E = c->E;
+  const int *iside = c->iside;
+  int status = 0;
+#pragma omp parallel for schedule(static) num_threads(c->nthreads)
+  for (int ie = 1; ie <= E; ie++) {
+    for (int j = 0; j < 3; j++) {
+      int code = c->elem_sides[3 * (size_t)(ie - 1) + j];
+      if (code == 0) continue;
+      int is = code > 0 ? code : -code;
+      int iel = ISIDE(3, is), ier = ISIDE(4, is);
+      int inl1 = ISIDE(5, is), inl2 = ISIDE(6, is);
+      int inr1 = ISIDE(7, is), inr2 = ISIDE(8, is);
+      if (ier == 0) {
+        int st = getbou(c, is, iv, ux, uy);
+        if (st) {
+#pragma omp critical
+          status = st;
+        }
+        continue;
+      }
+      double scpr = ux * c->nx[is - 1] + uy * c->ny[is - 1];
+      double edgun[2];
+      if (scpr > 0.0) {
+        edgun[0] = F(c, inl1, iv, iel) - c->umean[iel - 1];
+        edgun[1] = F(c, inl2, iv, iel) - c->umean[iel - 1];
+      } else {
+        edgun[0] = F(c, inr1, iv, ier) - c->umean[ier - 1];
+        edgun[1] = F(c, inr2, iv, ier) - c->umean[ier - 1];
+      }
+      double fx[2], fy[2], fn[2], rhsi[2];
+      getflu2(edgun, ux, uy, fx, fy);
+      double alen = (1.0 / 6.0) * c->el[is - 1];
+      fn[0] = c->nx[is - 1] * fx[0] + c->ny[is - 1] * fy[0];
+      fn[1] = c->nx[is - 1] * fx[1] + c->ny[is - 1] * fy[1];
+      edge_rhsi(fn, alen, rhsi);
+      if (code < 0) {
+        c->rhs[NE(inr1, ier)] = c->rhs[NE(inr1, ier)] - rhsi[0];
+        c->rhs[NE(inr2, ier)] = c->rhs[NE(inr2, ier)] - rhsi[1];
+      } else {
+        c->rhs[NE(inl1, iel)] = c->rhs[NE(inl1, iel)] + rhsi[0];
+        c->rhs[NE(inl2, iel)] = c->rhs[NE(inl2, iel)] + rhsi[1];
+      }
+    }
+  }
+  return status;
+}
+
+/* Threads for ora_step (1 = the serial restatement).  The threaded step
+ * computes the same values bit for bit; it is the all-core CPU baseline.
+ * Returns 0, or -100 when a table cannot be allocated. */
+int ora_set_threads(ora_ctx *c, int nthreads)
+{
+  c->nthreads = nthreads > 1 ? nthreads : 1;
+  if (c->nthreads == 1 || c->elem_sides) return 0;
+  const size_t E = c->E;
+  const int *iside = c->iside;
+  c->elem_sides = calloc(3 * E, sizeof(int));
+  c->node_si = calloc(3 * E, sizeof(double));
+  c->node_su = calloc(3 * E, sizeof(double));
+  c->node_sv = calloc(3 * E, sizeof(double));
+  c->node_sp = calloc(3 * E, sizeof(double));
+  if (!c->elem_sides || !c->node_si || !c->node_su || !c->node_sv || !c->node_sp) {
+    c->nthreads = 1;
+    return -100;
+  }
+  int *cnt = calloc(E + 1, sizeof(int));
+  if (!cnt) { c->nthreads = 1; return -100; }
+  for (int is = 1; is <= c->S; is++) { /* increasing side number */
+    int iel = ISIDE(3, is), ier = ISIDE(4, is);
+    if (cnt[iel] < 3) c->elem_sides[3 * (size_t)(iel - 1) + cnt[iel]++] = is;
+    if (ier > 0 && cnt[ier] < 3) c->elem_sides[3 * (size_t)(ier - 1) + cnt[ier]++] = -is;
+  }
+  free(cnt);
+  return 0;
+}
+
 /* One ordinate of the velocity loop (src/advnce.f90:286-377). */
 static int ordinate(ora_ctx *c, int iv, double dte)
 {
@@ -648,9 +806,11 @@ static int ordinate(ora_ctx *c, int iv, double dte)
   const double *geome = c->geome;
   double ux = c->cxa[iv - 1], uy = c->cya[iv - 1];
   const double c13 = 1. / 3.;
+  const int nt = c->nthreads > 1 ? c->nthreads : 1;
   /* GTEQNF */
-  ora_gteqnf(E, c->disnd, c->disux, c->disuy, c->disps, ux, uy, c->prm.m, c->nfo);
+  gteqnf_n(nt, E, c->disnd, c->disux, c->disuy, c->disps, ux, uy, c->prm.m, c->nfo);
   /* GETFLA (src/getfla.f90:16-30) */
+#pragma omp parallel for schedule(static) if (nt > 1) num_threads(nt)
   for (int ie = 1; ie <= E; ie++)
     for (int in = 1; in <= 3; in++) {
       double fv = F(c, in, iv, ie);
@@ -659,6 +819,7 @@ static int ordinate(ora_ctx *c, int iv, double dte)
       else { c->fluxp[k] = ux * fv; c->fluyp[k] = uy * fv; }
     }
   /* GETELC (src/getelc.f90:87-176) */
+#pragma omp parallel for schedule(static) if (nt > 1) num_threads(nt)
   for (int ie = 1; ie <= E; ie++) {
     double area = 0.5 * GEOME(7, ie);
     double unkel[3];
@@ -700,6 +861,11 @@ static int ordinate(ora_ctx *c, int iv, double dte)
   }
   /* RFILLA(DELUN) */
   for (size_t k = 0; k < 3 * (size_t)E; k++) c->delun[k] = 0.0;
+  if (nt > 1) {
+    int st = edges_threaded(c, iv, ux, uy);
+    if (st) return st;
+    goto edges_done;
+  }
   /* EDGFLXOPT (src/edgflxopt.f90:72-125) */
   for (int is = 1; is <= S; is++) {
     int iel = ISIDE(3, is), ier = ISIDE(4, is);
@@ -733,11 +899,15 @@ static int ordinate(ora_ctx *c, int iv, double dte)
     c->rhs[NE(inl1, iel)] = c->rhs[NE(inl1, iel)] + rhsi[0];
     c->rhs[NE(inl2, iel)] = c->rhs[NE(inl2, iel)] + rhsi[1];
   }
+edges_done:
   if (g_stage) memcpy(g_stage->rhs_edge, c->rhs, 3 * (size_t)E * sizeof(double));
   /* RHS *= DTE (src/advnce.f90:342-346) */
+#pragma omp parallel for schedule(static) if (nt > 1) num_threads(nt)
   for (size_t k = 0; k < 3 * (size_t)E; k++) c->rhs[k] = dte * c->rhs[k];
-  /* GETINC (src/getinc.f90:22-43) */
+  /* GETINC (src/getinc.f90:22-43); the largest |increment| (a max, exact in
+   * any order) */
   double res = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : res) if (nt > 1) num_threads(nt)
   for (int ie = 1; ie <= E; ie++)
     for (int in = 1; in <= 3; in++) {
       size_t k = NE(in, ie);
@@ -745,12 +915,13 @@ static int ordinate(ora_ctx *c, int iv, double dte)
       double coll = c->mu[k] * (c->nfo[k] - nf);
       double r1 = c->mmat[k] * c->rhs[k] + coll * dte;
       if (c->prm.mutant == ORA_MUTANT_RES_AFTER && c->inflow_node[k]) r1 = 0.0;
-      if (fabs(r1) > fabs(res)) res = fabs(r1);
+      if (fabs(r1) > res) res = fabs(r1);
       c->delun[k] = c->delun[k] + c->mmat[k] * c->rhs[k] + coll * dte;
     }
   c->residual[iv - 1] = res;
   if (g_stage) memcpy(g_stage->delun, c->delun, 3 * (size_t)E * sizeof(double));
   /* ADTHEM (src/adthem.f90:17-26) */
+#pragma omp parallel for schedule(static) if (nt > 1) num_threads(nt)
   for (int ie = 1; ie <= E; ie++)
     for (int in = 1; in <= 3; in++) {
       double v = F(c, in, iv, ie) + c->delun[NE(in, ie)];
@@ -773,6 +944,7 @@ static int ordinate(ora_ctx *c, int iv, double dte)
           }
       }
     } else {
+#pragma omp parallel for schedule(static) if (nt > 1) num_threads(nt)
       for (int ie = 1; ie <= E; ie++)
         for (int in = 1; in <= 3; in++)
           if (c->inflow_node[NE(in, ie)]) F(c, in, iv, ie) = nf;
@@ -987,7 +1159,8 @@ void ora_destroy(ora_ctx *c)
                c->uvel_pp, c->vvel_pp, c->ps_pp, c->temp_pp, c->flag, c->nd,
                c->rho, c->uvel, c->vvel, c->ps, c->temp, c->nfo, c->fluxp,
                c->fluyp, c->umean, c->rhs, c->delun, c->residual,
-               c->bou_of_side, c->flu_in, c->inflow_node};
+               c->bou_of_side, c->flu_in, c->inflow_node, c->elem_sides,
+               c->node_si, c->node_su, c->node_sv, c->node_sp};
   for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++) free(p[i]);
   free(c);
 }

@@ -84,6 +84,11 @@ typedef struct ora_step_info {
 
 int ora_step(ora_ctx *c, int nsteps, ora_step_info *info);
 
+/* Threads for ora_step (default 1, the serial restatement).  More threads
+ * give the same values bit for bit (per-node sums and each element's edge
+ * terms keep the serial order); used for the all-core CPU baseline. */
+int ora_set_threads(ora_ctx *c, int nthreads);
+
 /* DIS* node fields (3,E) and point fields (P) from the most recent MACROS. */
 void ora_get_node_fields(const ora_ctx *c, double *disnd, double *disux,
                          double *disuy, double *disps, double *mu);

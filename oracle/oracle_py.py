@@ -56,6 +56,8 @@ def lib():
         L.ora_set_f.argtypes = [C.c_void_p, _dp]
         L.ora_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(OraStepInfo)]
         L.ora_step.restype = C.c_int
+        L.ora_set_threads.argtypes = [C.c_void_p, C.c_int]
+        L.ora_set_threads.restype = C.c_int
         L.ora_get_node_fields.argtypes = [C.c_void_p] + [_dp] * 5
         L.ora_get_point_fields.argtypes = [C.c_void_p] + [_dp] * 6
         L.ora_get_eta.argtypes = [C.c_void_p, _dp]
@@ -138,6 +140,11 @@ class OracleRun:
             raise RuntimeError(f"ora_step failed with status {st}")
         return [dict(sumres=i.sumres, logres=i.logres, dte=i.dte, mass=i.mass,
                      lift=i.lift, drag=i.drag) for i in info]
+
+    def set_threads(self, n: int):
+        """Threads for step() (the same values bit for bit as one thread)."""
+        if lib().ora_set_threads(self.h, int(n)) != 0:
+            raise MemoryError("ora_set_threads: table allocation failed")
 
     def get_f(self) -> np.ndarray:
         """f in the reference layout, shape (E, V, 3)."""

@@ -1,0 +1,157 @@
+"""The two large configurations at their full size on the GPU.
+
+- Cylinder, 101,753 triangles x 16 x 32 ordinates (Mach 0.5, Kn 0.05): two
+  FP64 steps against the oracle (threaded: the same values bit for bit as
+  its serial order) on f, node and point fields, DTE, SUMRES, mass and the
+  forces; then 2000 steps in each form with property checks (finite, f >= 0,
+  bounded per-step mass change, finite wall parameter) and the FP32-bulk
+  form's point fields against the FP64 form's.
+- Particle, 1,005,669 triangles x 32 x 64 ordinates (Mach 0.3, Kn 1): 20
+  steps in each form with the same property checks, f checked on the device
+  (dgbgk_f_stats, no 49 GB host copy); and the same mesh with 2 x 64
+  ordinates against the oracle over two steps, which covers every element,
+  side and wall of the full mesh.
+Each context is destroyed before the next is created.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from small_cases import rel_l2, rel_l2_vec
+
+pytestmark = pytest.mark.gpu
+THREADS = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def cylinder():
+    from dgbgk import cases
+    return cases.cylinder_flow()
+
+
+@pytest.fixture(scope="module")
+def particle():
+    from dgbgk import cases
+    return cases.particle_drag()
+
+
+def _properties(s, recs, label, mass_step=1e-3):
+    """Finite records, f >= 0 and finite everywhere, bounded per-step mass
+    change, finite non-negative wall parameter, positive density."""
+    for k in ("sumres", "dte", "mass", "lift", "drag", "change"):
+        v = np.array([r[k] for r in recs])
+        assert np.all(np.isfinite(v)), (label, k)
+    m = np.array([recs[0]["mass"]] + [r["mass"] for r in recs])
+    dm = float(np.max(np.abs(np.diff(m)) / m[:-1]))
+    st = s.f_stats()
+    eta = s.wall_eta()
+    nf = s.node_fields()
+    _say(f"{label}: f in [{st['min']:.3e}, {st['max']:.3e}], non-finite {st['nonfinite']}, "
+         f"max per-step mass change {dm:.2e}, drag {recs[-1]['drag']:.6f}")
+    assert st["nonfinite"] == 0 and st["min"] >= 0.0, (label, st)
+    assert dm < mass_step, (label, dm)
+    assert np.all(np.isfinite(eta)) and np.all(eta >= 0.0), label
+    assert np.all(np.isfinite(nf["disnd"])) and np.all(nf["disnd"] > 0.0), label
+    return dm
+
+
+def test_cylinder_full_size_two_steps_against_oracle(cylinder):
+    from dgbgk.solver import Solver
+    from oracle_py import OracleRun
+    case = cylinder
+    _say(f"cylinder E={case.mesh.nelem} V={case.nvel} FP64: 2 steps vs the oracle "
+         f"({THREADS} threads)")
+    with Solver(case) as s:
+        rs = s.step(2)
+        f, nf, pf = s.get_f(), s.node_fields(), s.point_fields()
+    o = OracleRun(case)
+    o.set_threads(THREADS)
+    ro = o.step(2)
+    g, onf, opf = o.get_f(), o.node_fields(), o.point_fields()
+    o.close()
+    err = rel_l2(f, g)
+    del f, g
+    _say(f"cylinder 2 steps: rel L2(f) = {err:.3e}")
+    assert err < 1e-10
+    for k in ("disnd", "disps", "mu"):
+        assert rel_l2(nf[k], onf[k]) < 1e-10, k
+    assert rel_l2_vec(nf["disux"], nf["disuy"], onf["disux"], onf["disuy"]) < 1e-10
+    for k in ("nd", "rho", "ps", "temp"):
+        assert rel_l2(pf[k], opf[k]) < 1e-10, k
+    for a, b in zip(rs, ro):
+        assert abs(a["dte"] - b["dte"]) <= 1e-12 * b["dte"]
+        assert abs(a["sumres"] - b["sumres"]) <= 1e-10 * b["sumres"]
+        assert abs(a["mass"] - b["mass"]) <= 1e-12 * b["mass"]
+        assert abs(a["drag"] - b["drag"]) <= 1e-10 * max(abs(b["drag"]), 1.0)
+        assert abs(a["lift"] - b["lift"]) <= 1e-10 * max(abs(b["lift"]), 1.0)
+
+
+def test_cylinder_full_size_2000_steps_both_forms(cylinder):
+    from dgbgk import _lib
+    from dgbgk.solver import Solver
+    case = cylinder
+    out = {}
+    for prec, name in ((_lib.DGBGK_FP64, "FP64"), (_lib.DGBGK_FP32, "FP32-bulk")):
+        _say(f"cylinder E={case.mesh.nelem} V={case.nvel} {name}: 2000 steps")
+        with Solver(case, precision=prec) as s:
+            recs = s.step(2000)
+            _properties(s, recs, f"cylinder {name} 2000 steps")
+            out[name] = (s.point_fields(), recs[-1])
+    a, b = out["FP32-bulk"][0], out["FP64"][0]
+    errs = {k: rel_l2(a[k], b[k]) for k in ("rho", "ps", "temp")}
+    errs["uv"] = rel_l2_vec(a["uvel"], a["vvel"], b["uvel"], b["vvel"])
+    drag = abs(out["FP32-bulk"][1]["drag"] - out["FP64"][1]["drag"]) / abs(out["FP64"][1]["drag"])
+    _say("cylinder 2000 steps, FP32-bulk vs FP64: " +
+         ", ".join(f"{k} {v:.2e}" for k, v in errs.items()) + f", drag {drag:.2e}")
+    assert max(errs.values()) < 1e-4 and drag < 1e-4
+
+
+def test_particle_full_size_20_steps_both_forms(particle):
+    from dgbgk import _lib
+    from dgbgk.solver import Solver
+    case = particle
+    out = {}
+    for prec, name in ((_lib.DGBGK_FP64, "FP64"), (_lib.DGBGK_FP32, "FP32-bulk")):
+        _say(f"particle E={case.mesh.nelem} V={case.nvel} {name}: 20 steps")
+        with Solver(case, precision=prec) as s:
+            recs = s.step(20)
+            _properties(s, recs, f"particle {name} 20 steps")
+            out[name] = s.point_fields()
+    a, b = out["FP32-bulk"], out["FP64"]
+    errs = {k: rel_l2(a[k], b[k]) for k in ("rho", "ps", "temp")}
+    errs["uv"] = rel_l2_vec(a["uvel"], a["vvel"], b["uvel"], b["vvel"])
+    _say("particle 20 steps, FP32-bulk vs FP64: " +
+         ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert max(errs.values()) < 1e-5
+
+
+def test_particle_mesh_against_oracle():
+    """The full particle mesh with 2 x 64 ordinates, two FP64 steps, against
+    the oracle: every element, side and wall of the 1M-triangle mesh."""
+    from dgbgk import cases
+    from dgbgk.solver import Solver
+    from oracle_py import OracleRun
+    case = cases.particle_drag(rorder=2, torder=64)
+    _say(f"particle mesh E={case.mesh.nelem} V={case.nvel} FP64: 2 steps vs the oracle")
+    with Solver(case) as s:
+        rs = s.step(2)
+        f, pf = s.get_f(), s.point_fields()
+    o = OracleRun(case)
+    o.set_threads(THREADS)
+    ro = o.step(2)
+    g, opf = o.get_f(), o.point_fields()
+    o.close()
+    err = rel_l2(f, g)
+    _say(f"particle mesh 2 steps: rel L2(f) = {err:.3e}")
+    assert err < 1e-10
+    for k in ("nd", "ps", "temp"):
+        assert rel_l2(pf[k], opf[k]) < 1e-10, k
+    for a, b in zip(rs, ro):
+        assert abs(a["dte"] - b["dte"]) <= 1e-12 * b["dte"]
+        assert abs(a["sumres"] - b["sumres"]) <= 1e-10 * b["sumres"]
+        assert abs(a["mass"] - b["mass"]) <= 1e-12 * b["mass"]

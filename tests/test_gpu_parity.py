@@ -399,8 +399,10 @@ def test_getmac_step_gathers_the_point_pressure():
     record takes lift and drag from the point pressure gathered in the step
     itself, the same as a fresh GETFOR on the state after the step; the
     steps between keep the last gathered pressure.  Checked with step 1000
-    launched on its own and inside a replayed 64-step graph."""
+    launched on its own and inside a replayed 64-step graph, and against
+    the oracle's step-999 and step-1000 records."""
     from dgbgk.solver import Solver
+    from oracle_py import OracleRun
     case = cylinder_small(elems=800)
     print(f"case cylinder E={case.mesh.nelem} 1000 steps, GETMAC step", flush=True)
     s = Solver(case)
@@ -409,7 +411,14 @@ def test_getmac_step_gathers_the_point_pressure():
     s.close()
     a = rs[999]
     assert abs(a["drag"] - drag) <= 1e-12 * max(abs(drag), 1.0)
-    assert abs(a["lift"] - lift) <= 1e-12 * max(abs(drag), 1.0)
+    assert abs(a["lift"] - lift) <= 1e-12 * max(abs(lift), 1.0)
+    o = OracleRun(case)
+    o.set_threads(8)  # the same values bit for bit as one thread
+    ro = o.step(1000)
+    o.close()
+    for k in (998, 999):
+        assert abs(rs[k]["drag"] - ro[k]["drag"]) <= 1e-10 * max(abs(ro[k]["drag"]), 1.0), k
+        assert abs(rs[k]["lift"] - ro[k]["lift"]) <= 1e-10 * max(abs(ro[k]["lift"]), 1.0), k
     assert rs[998]["drag"] == rs[0]["drag"]  # stale since the last gather
     assert rs[998]["drag"] != a["drag"]
     g = Solver(case)
