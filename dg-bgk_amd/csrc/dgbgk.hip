@@ -141,7 +141,7 @@ __device__ __forceinline__ void nsum(double& s, double& c, double x) {
 // ---------------------------------------------------------------------------
 // k_update: one block = one tile of <= 256 elements (owned + halo) for a
 // chunk of ordinates.  Per ordinate: phase A computes every slot's predictor
-// (GTEQNF + GETELC's UMEAN, src/getelc.f90:95-138) and stages f and U in
+// (GTEQNF + GETELC's UMEAN, src/getelc.f90:54-83) and stages f and U in
 // LDS; phase B gives each owned element its volume term, its upwind edge
 // fluxes (element-centric EDGFLXOPT: each element recomputes the flux of its
 // own sides, so there is no scatter and no atomic), GETINC, ADTHEM and
@@ -210,7 +210,7 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
   store4(sf + 4 * slot, h[0], h[1], h[2], T(0));
   __syncthreads();
   if (!s.owned) return;
-  // volume term (src/getelc.f90:142-172): area * grad N_a . c (mean of f - U)
+  // volume term (src/getelc.f90:87-117): area * grad N_a . c (mean of f - U)
   const T sm = c13 * (h[0] + h[1] + h[2]);
   const T fx = ux * sm, fy = uy * sm;
   T rhs[3];
@@ -245,8 +245,8 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
     store4(wp + 4, cold[1], cold[2], U, T(0));
     return;
   }
-  // GETINC + ADTHEM + INFLOW (src/getinc.f90:30-42, src/adthem.f90:109-118,
-  // src/inflow.f90:165-181); mdt = 6/RJ * DTE
+  // GETINC + ADTHEM + INFLOW (src/getinc.f90:30-42, src/adthem.f90:17-26,
+  // src/inflow.f90:43-59); mdt = 6/RJ * DTE
   T* fo = a.fnxt + (size_t)v * a.ldf + 3 * (size_t)s.e;
   T d[3], fn[3], cg[3];
   // FP32: the dithered store with one dither fraction for the element's
@@ -813,7 +813,7 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
     a.mom[2 * N3 + k] = uy;
     a.mom[3 * N3 + k] = ps;
     a.mom[4 * N3 + k] = mu;
-    // GTEQNF parameters (src/gteqnf.f90:420-429)
+    // GTEQNF parameters (src/gteqnf.f90:28-37)
     double pa = 0.0, pb = 0.0;
     if (!(ps < 1e-20)) {
       pb = (a.mm * nd) / (2 * ps);

@@ -207,7 +207,7 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
     p.nx[s - 1] = dy / p.el[s - 1];
     p.ny[s - 1] = -dx / p.el[s - 1];
   }
-  // GETGEO (src/getgeo.f90:58-95)
+  // GETGEO (src/getgeo.f90:22-59)
   p.geome.resize(7 * (size_t)E);
   static const double nxi[3] = {-1.0, 1.0, 0.0}, net[3] = {-1.0, 0.0, 1.0};
   for (int e = 0; e < E; ++e) {
@@ -230,7 +230,7 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
     }
     g[6] = rj;
   }
-  // GETLEN (src/getlen.f90:114-123), GETMAT (src/getmat.f90:134-166)
+  // GETLEN (src/getlen.f90:17-26), GETMAT (src/getmat.f90:8-40)
   p.relen.resize(E);
   p.mmat.resize(3 * (size_t)E);
   for (int e = 0; e < E; ++e) {
@@ -245,7 +245,7 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
       p.mmat[3 * (size_t)e + a] = 1. / m;
     }
   }
-  // GETNOR (src/getnor.f90:190-200)
+  // GETNOR (src/getnor.f90:21-31)
   p.rsido.resize(3 * (size_t)B);
   for (int b = 1; b <= B; ++b) {
     int p0 = BS(1, b), p1 = BS(2, b);
@@ -255,7 +255,7 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
     p.rsido[3 * (size_t)(b - 1) + 1] = -dx * rt;
     p.rsido[3 * (size_t)(b - 1) + 2] = rl;
   }
-  // VSPACE (src/vspace.f90:52-86)
+  // VSPACE (src/vspace.f90:16-49)
   p.vcord.resize(3 * (size_t)V);
   {
     double gap = 2.0 / (double)T, s = 0.0;
@@ -270,7 +270,7 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
       }
     p.sumweight = s;
   }
-  // ALOTIM with MXSPD = rv (src/alotim.f90:365-371, src/advnce.f90:133)
+  // ALOTIM with MXSPD = rv (src/alotim.f90:12-18, src/advnce.f90:133)
   for (int e = 0; e < E; ++e) {
     double d = (prm.csafm * p.relen[e]) / (prm.rv);
     if (e == 0 || d < p.dtec) p.dtec = d;
@@ -289,7 +289,7 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
     p.cxb[v] = rb * std::cos(th); p.cyb[v] = rb * std::sin(th);
     p.jac[v] = kPi * prm.rv * prm.rv * 0.25 * (et + 1);
     p.wv[v] = p.vcord[3 * (size_t)v + 2] * p.jac[v] * frac;
-    // INFLOW value (src/inflow.f90:150-161)
+    // INFLOW value (src/inflow.f90:28-39)
     double tin = cinf[0], pin = cinf[1] * 100000.0;
     double rho = pin / (prm.rgas * tin), beta = std::sqrt(rho / (2 * pin));
     double c0 = (rho * kNa) / prm.m, c1 = (beta * beta) / kPi;

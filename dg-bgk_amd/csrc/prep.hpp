@@ -9,7 +9,7 @@
 
 namespace dgbgk {
 
-// Reference constants (e.g. src/advnce.f90:94, src/gteqnf.f90:410).
+// Reference constants (e.g. src/advnce.f90:94, src/gteqnf.f90:18).
 constexpr double kPi = 3.1416;
 constexpr double kNa = 6.022e26;
 

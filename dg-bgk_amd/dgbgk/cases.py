@@ -7,7 +7,7 @@ mass density; T = 1 and P = 1e-5 bar (P enters multiplied by 10**5) give
 rho = p = 1.  The thermal speed sqrt(RT) is then 1, the 2-D monatomic sound
 speed is sqrt(gamma R T) with gamma = 2, and Kn enters through the
 hard-sphere diameter d = (sqrt(2) * 3.1416 * n * Kn * L)**-0.5 that makes the
-reference's collision frequency (src/compmu.f90:295,341-342) give a mean
+reference's collision frequency (src/compmu.f90:45,90-92) give a mean
 free path of Kn * L.
 
 Radial quadrature.  The reference normalises every velocity sum by

@@ -61,7 +61,7 @@ def read_deck(path: str) -> dict:
 
 
 def read_lobatto(path: str):
-    """Line 1: RORDER; then RORDER lines 'eta weight' (src/vspace.f90:52-54)."""
+    """Line 1: RORDER; then RORDER lines 'eta weight' (src/vspace.f90:16-18)."""
     toks = open(path).read().split()
     r = int(float(toks[0]))
     vals = np.array([_fnum(t) for t in toks[1:1 + 2 * r]], dtype=np.float64).reshape(r, 2)
@@ -77,7 +77,7 @@ def write_lobatto(path: str, eta, w):
 
 def read_plt(path: str) -> Mesh:
     """Mesh file read as in src/spectralbgk.f90:240-272 and
-    src/gtinpt.f90:113-139: NLINES, NLINES title lines, a text line, NDIMN,
+    src/gtinpt.f90:24-50: NLINES, NLINES title lines, a text line, NDIMN,
     a text line, 'NELEM NPOIN NBOUN', then the connectivity, coordinate and
     boundary-side blocks, each after one header line."""
     with open(path) as fh:

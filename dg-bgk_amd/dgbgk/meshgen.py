@@ -1,12 +1,12 @@
 """Synthetic meshes in the reference's conventions.
 
 A mesh is the triple the reference reads from its ``.plt`` file
-(src/gtinpt.f90:113-139): ``intma`` (E,3) 1-based point ids of each linear
+(src/gtinpt.f90:24-50): ``intma`` (E,3) 1-based point ids of each linear
 triangle, ``coord`` (P,2), and ``bsido`` (B,6) boundary sides
 ``p1 p2 element type Tw alpha*100``.  Elements are counter-clockwise and every
 boundary side has the domain on its left, the orientation the reference's
 side builder and normals assume (src/getsid.f90:85-129, src/normal.f90:27-31,
-src/getnor.f90:190-200).  Boundary types: 1 inflow, 2 outflow, 3 wall
+src/getnor.f90:21-31).  Boundary types: 1 inflow, 2 outflow, 3 wall
 without the mass-flux parameter, 4 wall with it.
 
 Elements and points are numbered along a Hilbert curve so neighbouring

@@ -14,7 +14,7 @@ Steady means: the step change max|f^{k+1} - f^k| has fallen below tol
 times the step-1 change of the FP64 freestream run of the same case.  The
 reference's own residual (SUMRES, src/advnce.f90:421-429) cannot serve:
 it counts the increment at nodes that INFLOW then resets to the inflow
-state (src/inflow.f90:165-181), so it levels off, and it is normalised by
+state (src/inflow.f90:43-59), so it levels off, and it is normalised by
 the run's own first step, which a warm start would shrink.
 
 The box sides are inflow (type 1) rather than outflow (type 2): the

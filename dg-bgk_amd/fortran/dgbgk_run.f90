@@ -5,8 +5,8 @@
 !
 ! Reads the reference's run deck (namelist /Input/ IVD, the same keys and
 ! defaults as src/spectralbgk.f90:116-148), its velocity file
-! (src/spectralbgk.f90:157-161, src/vspace.f90:52-54) and its mesh file
-! (src/spectralbgk.f90:240-272, src/gtinpt.f90:113-139); writes FILE.OUT,
+! (src/spectralbgk.f90:157-161, src/vspace.f90:16-18) and its mesh file
+! (src/spectralbgk.f90:240-272, src/gtinpt.f90:24-50); writes FILE.OUT,
 ! RESIDUAL.RES and RESULTS1/RESULTS2.RES in the reference's layouts
 ! (src/advnce.f90:103-120,163-170,403-435,468-489).  The time march itself
 ! runs on the GPU through the C ABI (module DGBGK_API).  --check builds the

@@ -83,7 +83,7 @@ typedef struct dgbgk_ctx dgbgk_ctx;
  * VSPACE, ALOTIM (src/dg_lin_convec.f90:96-123, src/spectralbgk.f90:205,
  * src/advnce.f90:133), the INICON2 Maxwellian seed (src/inicon2.f90:120-193)
  * and the initial MACROS (src/advnce.f90:139-160).  eta/w are the radial
- * quadrature (the Lobatto file's two columns, src/vspace.f90:52-61).
+ * quadrature (the Lobatto file's two columns, src/vspace.f90:16-25).
  * Every per-element array of this interface is in the caller's element
  * numbering; the device stores elements in Hilbert order of their
  * centroids (DGBGK_REORDER=0 in the environment keeps the input order). */
@@ -110,7 +110,7 @@ int dgbgk_seed_maxwellian(dgbgk_ctx *ctx, const double *rho, const double *u,
                           const double *v, const double *temp);
 
 /* MACROS node fields (3,NELEM): DISND, DISUX, DISUY, DISPS and COMPMU's MU
- * (src/macros.f90:126-164, src/compmu.f90:336-347) for the current f. */
+ * (src/macros.f90:126-164, src/compmu.f90:86-97) for the current f. */
 int dgbgk_node_fields(dgbgk_ctx *ctx, double *disnd, double *disux,
                       double *disuy, double *disps, double *mu);
 

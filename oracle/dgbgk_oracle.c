@@ -21,7 +21,7 @@
  *  - conditions on which the reference STOPs, or reads out of bounds, return
  *    a negative status instead.
  * Vacuum nodes (DISND < 1e-20) leave COMPMU's CT uninitialised in the
- * reference (src/compmu.f90:306-326); here CT = 0 there (parity unpinned).
+ * reference (src/compmu.f90:55-76); here CT = 0 there (parity unpinned).
  */
 #include "dgbgk_oracle.h"
 
@@ -30,7 +30,7 @@
 #include <string.h>
 
 #define PI_R 3.1416     /* PARAMETER(PI=3.1416), e.g. src/advnce.f90:94 */
-#define NA_R 6.022e26   /* PARAMETER(NA=6.022E+26), src/gteqnf.f90:410 */
+#define NA_R 6.022e26   /* PARAMETER(NA=6.022E+26), src/gteqnf.f90:18 */
 
 #define INTMA(j, e) intma[3 * ((size_t)(e)-1) + (j)-1]
 #define ISIDE(k, s) iside[8 * ((size_t)(s)-1) + (k)-1]
@@ -152,7 +152,7 @@ void ora_normal(int nside, const int *iside, const double *coord,
   }
 }
 
-/* GETGEO: shape-function gradients and RJ = 2*area (src/getgeo.f90:58-95). */
+/* GETGEO: shape-function gradients and RJ = 2*area (src/getgeo.f90:22-59). */
 void ora_getgeo(int nelem, const int *intma, const double *coord, double *geome)
 {
   static const double nxi[3] = {-1.0, 1.0, 0.0}, net[3] = {-1.0, 0.0, 1.0};
@@ -176,7 +176,7 @@ void ora_getgeo(int nelem, const int *intma, const double *coord, double *geome)
   }
 }
 
-/* GETLEN: minimum element altitude (src/getlen.f90:114-123). */
+/* GETLEN: minimum element altitude (src/getlen.f90:17-26). */
 void ora_getlen(int nelem, const double *geome, double *relen)
 {
   for (int ie = 1; ie <= nelem; ie++) {
@@ -192,7 +192,7 @@ void ora_getlen(int nelem, const double *geome, double *relen)
   }
 }
 
-/* GETMAT: inverse lumped mass, 1/(4*RJ/24) (src/getmat.f90:134-166). */
+/* GETMAT: inverse lumped mass, 1/(4*RJ/24) (src/getmat.f90:8-40). */
 void ora_getmat(int nelem, const double *geome, double *mmat)
 {
   const double c24 = 1. / 24., c4 = 4.0;
@@ -206,7 +206,7 @@ void ora_getmat(int nelem, const double *geome, double *mmat)
   }
 }
 
-/* GETNOR: boundary normals and lengths RSIDO(3,NBOUN) (src/getnor.f90:190-200). */
+/* GETNOR: boundary normals and lengths RSIDO(3,NBOUN) (src/getnor.f90:21-31). */
 void ora_getnor(int nboun, const int *bsido, const double *coord, double *rsido)
 {
   for (int ib = 1; ib <= nboun; ib++) {
@@ -221,7 +221,7 @@ void ora_getnor(int nboun, const int *bsido, const double *coord, double *rsido)
   }
 }
 
-/* VSPACE: ring-major ordinate table VCORD(3,VNPNT) (src/vspace.f90:52-86). */
+/* VSPACE: ring-major ordinate table VCORD(3,VNPNT) (src/vspace.f90:16-49). */
 void ora_vspace(int rorder, int torder, const double *eta, const double *w,
                 double *vcord, double *sumweight)
 {
@@ -282,7 +282,7 @@ double ora_sumweight(const ora_ctx *c) { return c->sumweight; }
 #define F(c, in, iv, ie) (c)->f[(((size_t)(ie)-1) * (c)->V + ((iv)-1)) * 3 + ((in)-1)]
 #define FS(c, in, iv, ie) (c)->fsnap[(((size_t)(ie)-1) * (c)->V + ((iv)-1)) * 3 + ((in)-1)]
 
-/* GTEQNF: local Maxwellian for one ordinate (src/gteqnf.f90:414-432). */
+/* GTEQNF: local Maxwellian for one ordinate (src/gteqnf.f90:26-40). */
 static void gteqnf_n(int nthreads, int nelem, const double *disnd, const double *disux,
                      const double *disuy, const double *disps, double ux, double uy,
                      double m, double *nfo)
@@ -311,7 +311,7 @@ void ora_gteqnf(int nelem, const double *disnd, const double *disux,
   gteqnf_n(1, nelem, disnd, disux, disuy, disps, ux, uy, m, nfo);
 }
 
-/* Inflow Maxwellian f_inf(c) (src/inflow.f90:150-161). */
+/* Inflow Maxwellian f_inf(c) (src/inflow.f90:28-39). */
 double ora_inflow_value(const double *cinf, double rgas, double m, double ux,
                         double uy)
 {
@@ -484,7 +484,7 @@ static void getmac(ora_ctx *c)
   memcpy(c->temp, c->temp_pp, n);
 }
 
-/* COMPMU: collision frequency per DG node (src/compmu.f90:286-347). */
+/* COMPMU: collision frequency per DG node (src/compmu.f90:36-97). */
 static void compmu(ora_ctx *c)
 {
   const int E = c->E, V = c->V;
@@ -619,7 +619,7 @@ static int reflect_lookup(const ora_ctx *c, double ux, double uy, double anx,
   return 0;
 }
 
-/* GETFLU on a 2-vector (src/getflu.f90:41-52). */
+/* GETFLU on a 2-vector (src/getflu.f90:7-18). */
 static inline void getflu2(const double u[2], double ux, double uy, double fx[2],
                            double fy[2])
 {
@@ -818,7 +818,7 @@ static int ordinate(ora_ctx *c, int iv, double dte)
       if (fabs(fv) == 0.0) { c->fluxp[k] = 0.0; c->fluyp[k] = 0.0; }
       else { c->fluxp[k] = ux * fv; c->fluyp[k] = uy * fv; }
     }
-  /* GETELC (src/getelc.f90:87-176) */
+  /* GETELC (src/getelc.f90:40-121) */
 #pragma omp parallel for schedule(static) if (nt > 1) num_threads(nt)
   for (int ie = 1; ie <= E; ie++) {
     double area = 0.5 * GEOME(7, ie);
@@ -928,7 +928,7 @@ edges_done:
       if (v < 0.0) v = 0.0;
       F(c, in, iv, ie) = v;
     }
-  /* INFLOW (src/inflow.f90:150-181) */
+  /* INFLOW (src/inflow.f90:28-59) */
   int have_inflow = 0;
   for (int ib = 1; ib <= c->B; ib++) if (BSIDO(4, ib) == 1) { have_inflow = 1; break; }
   if (have_inflow) {
@@ -1067,7 +1067,7 @@ int ora_create(int nelem, int npoin, int nboun, const int *intma_in,
   ora_getmat(nelem, c->geome, c->mmat);
   ora_getnor(nboun, c->bsido, c->coord, c->rsido);
   ora_vspace(rorder, torder, eta, w, c->vcord, &c->sumweight);
-  /* ALOTIM (src/alotim.f90:365-371) with MXSPD = rv */
+  /* ALOTIM (src/alotim.f90:12-18) with MXSPD = rv */
   c->dtec = 0.0;
   for (size_t e = 0; e < E; e++) {
     double d = (prm->csafm * c->relen[e]) / (prm->rv);

@@ -143,7 +143,7 @@ def test_interior_fluxes_cancel_pairwise():
 
 
 def test_inflow_value_closed_form():
-    """INFLOW's Maxwellian (src/inflow.f90:150-161) in the non-dimensional
+    """INFLOW's Maxwellian (src/inflow.f90:28-39) in the non-dimensional
     deck: rho = p = 1, so f_inf(c) = (1/(2 pi_ref)) exp(-|c-u|^2/2)."""
     from oracle_py import lib
     cinf = np.array([1.0, 1e-5, 0.3, -0.2])

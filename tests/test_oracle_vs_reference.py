@@ -131,7 +131,7 @@ def test_gteqnf_and_getfor_bit_exact():
     ux = rng.standard_normal(3 * E) * 0.3
     uy = rng.standard_normal(3 * E) * 0.3
     ps = rng.random(3 * E) + 0.5
-    ps[::17] = 1e-25  # vacuum branch (src/gteqnf.f90:424)
+    ps[::17] = 1e-25  # vacuum branch (src/gteqnf.f90:32)
     for (cx, cy) in [(0.3, -1.2), (2.5, 0.1)]:
         a, b = np.zeros(3 * E), np.zeros(3 * E)
         lib().ora_gteqnf(E, nd, ux, uy, ps, cx, cy, 6.022e26, a)
