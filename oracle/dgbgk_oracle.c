@@ -26,6 +26,7 @@
 #include "dgbgk_oracle.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -40,6 +41,17 @@
 #define RSIDO(k, b) rsido[3 * ((size_t)(b)-1) + (k)-1]
 #define COORD(k, p) coord[2 * ((size_t)(p)-1) + (k)-1]
 #define NE(a, e) (3 * ((size_t)(e)-1) + (a)-1) /* (NNODE,NELEM) arrays */
+
+/* FP32 emulation of chosen stages of the step (ora_set_emulation; tests and
+ * precision studies only, serial step only).  A(x) rounds one arithmetic
+ * result to FP32 when the step's arithmetic is emulated. */
+static int g_emu = 0;
+static inline double emu_round(int flag, double x) { return (g_emu & flag) ? (double)(float)x : x; }
+#define A(x) emu_round(ORA_EMU_ARITH, (x))
+#define AP(x) emu_round(ORA_EMU_ARITH | ORA_EMU_PRED, (x))
+#define AE(x) emu_round(ORA_EMU_ARITH | ORA_EMU_EDGE, (x))
+#define AI(x) emu_round(ORA_EMU_ARITH | ORA_EMU_INC, (x))
+#define P(x) emu_round(ORA_EMU_PARAMS, (x))
 
 /* ------------------------------------------------------------------------ */
 /* GETSID: side table ISIDE(8,NSIDE) (src/getsid.f90:1-216).                 */
@@ -267,6 +279,7 @@ struct ora_ctx {
   /* threaded stepping (ora_set_threads): each element's sides in increasing
    * side number, +is for the left element, -is for the right one */
   int nthreads;
+  int emul; /* ORA_EMU_* stages emulated in FP32 (precision studies) */
   int *elem_sides;
   double *node_si, *node_su, *node_sv, *node_sp;
 };
@@ -299,6 +312,12 @@ static void gteqnf_n(int nthreads, int nelem, const double *disnd, const double 
       if (ps < 1e-20) continue;
       double cx = disux[k], cy = disuy[k];
       double beta2 = rho / (2 * ps);
+      if (g_emu & ORA_EMU_PARAMS) { /* FP32 parameters, FP32 exponential */
+        const float pa = (float)(nd * (beta2 / PI_R)), pb = (float)beta2;
+        const float dx = (float)cx - (float)ux, dy = (float)cy - (float)uy;
+        nfo[k] = (double)(pa * expf(-pb * (dx * dx + dy * dy)));
+        continue;
+      }
       double tmp = -beta2 * ((cx - ux) * (cx - ux) + (cy - uy) * (cy - uy));
       nfo[k] = nd * (beta2 / PI_R) * exp(tmp);
     }
@@ -625,7 +644,7 @@ static inline void getflu2(const double u[2], double ux, double uy, double fx[2]
 {
   for (int i = 0; i < 2; i++) {
     if (fabs(u[i]) == 0.0) { fx[i] = 0.0; fy[i] = 0.0; }
-    else { fx[i] = ux * u[i]; fy[i] = uy * u[i]; }
+    else { fx[i] = AE(ux * u[i]); fy[i] = AE(uy * u[i]); }
   }
 }
 
@@ -637,8 +656,8 @@ static inline void edge_rhsi(const double fn[2], double alen, double rhsi[2])
   rhsi[1] = 0.0;
   for (int in = 0; in < 2; in++)
     for (int jn = 0; jn < 2; jn++) {
-      double cm = fn[jn] * mm[in][jn] * alen;
-      rhsi[in] = rhsi[in] - cm;
+      double cm = AE(AE(fn[jn] * mm[in][jn]) * alen);
+      rhsi[in] = AE(rhsi[in] - cm);
     }
 }
 
@@ -658,32 +677,37 @@ static int getbou(ora_ctx *c, int is, int iv, double ux, double uy)
     isb = c->bou_of_side[is - 1];
     if (isb < 1) return -2;
   }
-  double alen = (1. / 6.) * RSIDO(3, isb);
-  double anx = RSIDO(1, isb), any = RSIDO(2, isb);
+  double alen = AE((1. / 6.) * RSIDO(3, isb));
+  double anx = AE(RSIDO(1, isb)), any = AE(RSIDO(2, isb));
   int index = BSIDO(4, isb);
   double temp = (double)BSIDO(5, isb);
   double *rhs = c->rhs;
   double unk1 = F(c, in1, iv, iel), unk2 = F(c, in2, iv, iel);
-  double unsid[2] = {unk1 - c->umean[iel - 1], unk2 - c->umean[iel - 1]};
+  double unsid[2] = {AE(unk1 - c->umean[iel - 1]), AE(unk2 - c->umean[iel - 1])};
   double fx[2], fy[2], fn[2], rhsi[2];
   getflu2(unsid, ux, uy, fx, fy);
-  fn[0] = anx * fx[0] + any * fy[0];
-  fn[1] = anx * fx[1] + any * fy[1];
-  double test = ux * anx + uy * any;
+  fn[0] = AE(AE(anx * fx[0]) + AE(any * fy[0]));
+  fn[1] = AE(AE(anx * fx[1]) + AE(any * fy[1]));
+  /* the branch and the reflection lookup on the FP64 values, also when the
+   * arithmetic is emulated in FP32: the reference's lookup is discontinuous
+   * at its bracket ends, and the library builds it on the host in FP64 */
+  const double uxd = c->cxa[iv - 1], uyd = c->cya[iv - 1];
+  const double anxd = RSIDO(1, isb), anyd = RSIDO(2, isb);
+  double test = uxd * anxd + uyd * anyd;
   if (index == 1) return 0; /* inflow side: no flux (:88-138) */
   if (index == 2 || test >= 0.0) {
     edge_rhsi(fn, alen, rhsi);
-    rhs[NE(in1, iel)] = rhs[NE(in1, iel)] + rhsi[0];
-    rhs[NE(in2, iel)] = rhs[NE(in2, iel)] + rhsi[1];
+    rhs[NE(in1, iel)] = AE(rhs[NE(in1, iel)] + rhsi[0]);
+    rhs[NE(in2, iel)] = AE(rhs[NE(in2, iel)] + rhsi[1]);
     return 0;
   }
   /* wall, molecules leaving the wall (:189-280) */
-  double speed = sqrt(ux * ux + uy * uy);
+  double speed = sqrt(uxd * uxd + uyd * uyd);
   double mw = exp(-(speed * speed) / (2 * c->prm.rgas * temp));
   double mf = mw * c->eta_b[isb - 1];
   int i1, i2;
   double zd1, zd2;
-  int st = reflect_lookup(c, ux, uy, anx, any, &i1, &i2, &zd1, &zd2);
+  int st = reflect_lookup(c, uxd, uyd, anxd, anyd, &i1, &i2, &zd1, &zd2);
   if (st) return st;
   double rf11, rf12, rf21, rf22;
   if (c->prm.jacobi) {
@@ -693,17 +717,18 @@ static int getbou(ora_ctx *c, int is, int iv, double ux, double uy)
     rf11 = F(c, in1, i1, iel); rf12 = F(c, in1, i2, iel);
     rf21 = F(c, in2, i1, iel); rf22 = F(c, in2, i2, iel);
   }
-  double rf1 = (zd2 * rf11 + zd1 * rf12) / (zd1 + zd2);
-  double rf2 = (zd2 * rf21 + zd1 * rf22) / (zd1 + zd2);
+  mf = AE(mf);
+  double rf1 = AE(AE(AE(zd2 * rf11) + AE(zd1 * rf12)) / AE(zd1 + zd2));
+  double rf2 = AE(AE(AE(zd2 * rf21) + AE(zd1 * rf22)) / AE(zd1 + zd2));
   double a = c->prm.alpha;
-  unsid[0] = (1 - a) * rf1 + a * mf;
-  unsid[1] = (1 - a) * rf2 + a * mf;
+  unsid[0] = AE(AE((1 - a) * rf1) + AE(a * mf));
+  unsid[1] = AE(AE((1 - a) * rf2) + AE(a * mf));
   getflu2(unsid, ux, uy, fx, fy);
-  fn[0] = anx * fx[0] + any * fy[0];
-  fn[1] = anx * fx[1] + any * fy[1];
+  fn[0] = AE(AE(anx * fx[0]) + AE(any * fy[0]));
+  fn[1] = AE(AE(anx * fx[1]) + AE(any * fy[1]));
   edge_rhsi(fn, alen, rhsi);
-  rhs[NE(in1, iel)] = rhs[NE(in1, iel)] + rhsi[0];
-  rhs[NE(in2, iel)] = rhs[NE(in2, iel)] + rhsi[1];
+  rhs[NE(in1, iel)] = AE(rhs[NE(in1, iel)] + rhsi[0]);
+  rhs[NE(in2, iel)] = AE(rhs[NE(in2, iel)] + rhsi[1]);
   return 0;
 }
 
@@ -798,15 +823,139 @@ int ora_set_threads(ora_ctx *c, int nthreads)
   return 0;
 }
 
+/* An FP32 store of f + d with a dither uniform over +-1/2 ulp of f added to
+ * d before the one rounding (stochastic rounding at f's exponent): the
+ * dither fraction is a hash of (element, ordinate, step), shared by the
+ * element's three nodes.  The caller clamps and rounds the result. */
+static double emu_store_sr(double f, double d, int ie, int iv, int itime)
+{
+  if (f == 0.0) return f + d;
+  uint32_t h = (uint32_t)ie * 2654435761u ^ (uint32_t)iv * 2246822519u ^
+               (uint32_t)itime * 3266489917u;
+  h ^= h >> 15; h *= 2246822519u; h ^= h >> 13; h *= 3266489917u; h ^= h >> 16;
+  const double r = (double)h / 4294967296.0;
+  int ex;
+  frexp((double)(float)f, &ex);                 /* f = m 2^ex, 0.5 <= m < 1 */
+  const double ulp = ldexp(1.0, ex - 24);        /* FP32 ulp of f */
+  return f + (d + (r - 0.5) * ulp);
+}
+
+/* The upwind values GETBOU takes on a boundary side (precision studies of
+ * the mean form): 0 on an inflow side (no flux), the element's own values
+ * on an outflow side or where c.n >= 0, else the wall's reflected
+ * distribution; the same branches and arithmetic as getbou(). */
+static int getbou_values(ora_ctx *c, int is, int iv, double ux, double uy, double g[2])
+{
+  const int *iside = c->iside, *bsido = c->bsido;
+  const double *rsido = c->rsido;
+  int iel = ISIDE(3, is), in1 = ISIDE(5, is), in2 = ISIDE(6, is);
+  int isb = c->bou_of_side[is - 1];
+  if (isb < 1) return -2;
+  int index = BSIDO(4, isb);
+  double temp = (double)BSIDO(5, isb);
+  const double uxd = c->cxa[iv - 1], uyd = c->cya[iv - 1];
+  const double anxd = RSIDO(1, isb), anyd = RSIDO(2, isb);
+  double test = uxd * anxd + uyd * anyd;
+  (void)ux; (void)uy;
+  if (index == 1) { g[0] = g[1] = 0.0; return 0; }
+  if (index == 2 || test >= 0.0) {
+    g[0] = AE(F(c, in1, iv, iel) - c->umean[iel - 1]);
+    g[1] = AE(F(c, in2, iv, iel) - c->umean[iel - 1]);
+    return 0;
+  }
+  double speed = sqrt(uxd * uxd + uyd * uyd);
+  double mw = exp(-(speed * speed) / (2 * c->prm.rgas * temp));
+  double mf = AE(mw * c->eta_b[isb - 1]);
+  int i1, i2;
+  double zd1, zd2;
+  int st = reflect_lookup(c, uxd, uyd, anxd, anyd, &i1, &i2, &zd1, &zd2);
+  if (st) return st;
+  double rf11 = F(c, in1, i1, iel), rf12 = F(c, in1, i2, iel);
+  double rf21 = F(c, in2, i1, iel), rf22 = F(c, in2, i2, iel);
+  double rf1 = AE(AE(AE(zd2 * rf11) + AE(zd1 * rf12)) / AE(zd1 + zd2));
+  double rf2 = AE(AE(AE(zd2 * rf21) + AE(zd1 * rf22)) / AE(zd1 + zd2));
+  double a = c->prm.alpha;
+  g[0] = AE(AE((1 - a) * rf1) + AE(a * mf));
+  g[1] = AE(AE((1 - a) * rf2) + AE(a * mf));
+  return 0;
+}
+
+/* The volume term folded into the edge terms (precision studies): since
+ * area grad N_k is half the sum of len n over the element's edges at node
+ * k, each edge of an element acts on (upwind values - the element's mean
+ * of f - U), an inflow side on (0 - mean).  Equal to the reference's
+ * operator in exact arithmetic; its terms are differences, so their
+ * rounding is relative to the variation of f instead of f. */
+static int edges_meanform(ora_ctx *c, int iv, double ux, double uy)
+{
+  const int E = c->E;
+  const int *iside = c->iside;
+  const double c13 = AE(1. / 3.);
+  if (!c->elem_sides && ora_set_threads(c, 2)) return -100;
+  c->nthreads = 1;
+  for (int ie = 1; ie <= E; ie++) {
+    const double gb = AE(c13 * AE(AE(AE(F(c, 1, iv, ie) - c->umean[ie - 1]) +
+                                     AE(F(c, 2, iv, ie) - c->umean[ie - 1])) +
+                                  AE(F(c, 3, iv, ie) - c->umean[ie - 1])));
+    for (int j = 0; j < 3; j++) {
+      int code = c->elem_sides[3 * (size_t)(ie - 1) + j];
+      if (code == 0) continue;
+      int is = code > 0 ? code : -code;
+      int iel = ISIDE(3, is), ier = ISIDE(4, is);
+      int inl1 = ISIDE(5, is), inl2 = ISIDE(6, is);
+      int inr1 = ISIDE(7, is), inr2 = ISIDE(8, is);
+      double g[2];
+      if (ier == 0) {
+        int st = getbou_values(c, is, iv, ux, uy, g);
+        if (st) return st;
+      } else if (ux * c->nx[is - 1] + uy * c->ny[is - 1] > 0.0) {
+        g[0] = AE(F(c, inl1, iv, iel) - c->umean[iel - 1]);
+        g[1] = AE(F(c, inl2, iv, iel) - c->umean[iel - 1]);
+      } else {
+        g[0] = AE(F(c, inr1, iv, ier) - c->umean[ier - 1]);
+        g[1] = AE(F(c, inr2, iv, ier) - c->umean[ier - 1]);
+      }
+      double h[2] = {AE(g[0] - gb), AE(g[1] - gb)};
+      double fx[2], fy[2], fn[2], rhsi[2];
+      getflu2(h, ux, uy, fx, fy);
+      const double snx = ier == 0 ? AE(c->rsido[3 * (c->bou_of_side[is - 1] - 1)]) : AE(c->nx[is - 1]);
+      const double sny = ier == 0 ? AE(c->rsido[3 * (c->bou_of_side[is - 1] - 1) + 1]) : AE(c->ny[is - 1]);
+      const double len = ier == 0 ? c->rsido[3 * (c->bou_of_side[is - 1] - 1) + 2] : c->el[is - 1];
+      double alen = AE((1.0 / 6.0) * len);
+      fn[0] = AE(AE(snx * fx[0]) + AE(sny * fy[0]));
+      fn[1] = AE(AE(snx * fx[1]) + AE(sny * fy[1]));
+      edge_rhsi(fn, alen, rhsi);
+      if (code > 0) {
+        c->rhs[NE(inl1, iel)] = AE(c->rhs[NE(inl1, iel)] + rhsi[0]);
+        c->rhs[NE(inl2, iel)] = AE(c->rhs[NE(inl2, iel)] + rhsi[1]);
+      } else {
+        c->rhs[NE(inr1, ier)] = AE(c->rhs[NE(inr1, ier)] - rhsi[0]);
+        c->rhs[NE(inr2, ier)] = AE(c->rhs[NE(inr2, ier)] - rhsi[1]);
+      }
+    }
+  }
+  return 0;
+}
+
+/* FP32 emulation of stages of the step (precision studies; serial steps
+ * only: the threaded step is switched off while it is set). */
+void ora_set_emulation(ora_ctx *c, int flags)
+{
+  c->emul = flags;
+  if (flags & (ORA_EMU_STORE_RN | ORA_EMU_STORE_SR))
+    for (size_t k = 0; k < 3 * (size_t)c->V * c->E; k++) c->f[k] = (double)(float)c->f[k];
+}
+
 /* One ordinate of the velocity loop (src/advnce.f90:286-377). */
 static int ordinate(ora_ctx *c, int iv, double dte)
 {
   const int E = c->E, S = c->S;
   const int *iside = c->iside, *intma = c->intma, *bsido = c->bsido;
   const double *geome = c->geome;
-  double ux = c->cxa[iv - 1], uy = c->cya[iv - 1];
-  const double c13 = 1. / 3.;
-  const int nt = c->nthreads > 1 ? c->nthreads : 1;
+  double ux = emu_round(ORA_EMU_ARITH | ORA_EMU_PRED | ORA_EMU_EDGE | ORA_EMU_INC, c->cxa[iv - 1]),
+         uy = emu_round(ORA_EMU_ARITH | ORA_EMU_PRED | ORA_EMU_EDGE | ORA_EMU_INC, c->cya[iv - 1]);
+  const double c13 = AP(1. / 3.);
+  const int nt = c->nthreads > 1 && !g_emu ? c->nthreads : 1;
   /* GTEQNF */
   gteqnf_n(nt, E, c->disnd, c->disux, c->disuy, c->disps, ux, uy, c->prm.m, c->nfo);
   /* GETFLA (src/getfla.f90:16-30) */
@@ -816,42 +965,50 @@ static int ordinate(ora_ctx *c, int iv, double dte)
       double fv = F(c, in, iv, ie);
       size_t k = NE(in, ie);
       if (fabs(fv) == 0.0) { c->fluxp[k] = 0.0; c->fluyp[k] = 0.0; }
-      else { c->fluxp[k] = ux * fv; c->fluyp[k] = uy * fv; }
+      else { c->fluxp[k] = AP(ux * fv); c->fluyp[k] = AP(uy * fv); }
     }
   /* GETELC (src/getelc.f90:40-121) */
 #pragma omp parallel for schedule(static) if (nt > 1) num_threads(nt)
   for (int ie = 1; ie <= E; ie++) {
-    double area = 0.5 * GEOME(7, ie);
+    double area = AP(0.5 * GEOME(7, ie));
     double unkel[3];
     double dive = 0.0, coll = 0.0;
     for (int in = 1; in <= 3; in++) unkel[in - 1] = F(c, in, iv, ie);
+    if (g_emu & ORA_EMU_FORM) { /* difference form: the gradients sum to zero */
+      const size_t k0 = NE(1, ie), k1 = NE(2, ie), k2 = NE(3, ie);
+      dive = AP(AP(AP(AP(GEOME(2, ie)) * AP(c->fluxp[k1] - c->fluxp[k0])) +
+                   AP(AP(GEOME(3, ie)) * AP(c->fluxp[k2] - c->fluxp[k0]))) +
+                AP(AP(AP(GEOME(5, ie)) * AP(c->fluyp[k1] - c->fluyp[k0])) +
+                   AP(AP(GEOME(6, ie)) * AP(c->fluyp[k2] - c->fluyp[k0]))));
+    } else
     for (int in = 1; in <= 3; in++) {
       size_t k = NE(in, ie);
-      dive = dive + GEOME(in, ie) * c->fluxp[k] + GEOME(in + 3, ie) * c->fluyp[k];
+      dive = AP(AP(dive + AP(AP(GEOME(in, ie)) * c->fluxp[k])) + AP(AP(GEOME(in + 3, ie)) * c->fluyp[k]));
     }
     for (int in = 1; in <= 3; in++) {
       size_t k = NE(in, ie);
-      coll = coll + c->mu[k] * (c->nfo[k] - F(c, in, iv, ie));
+      coll = AP(coll + AP(P(c->mu[k]) * AP(c->nfo[k] - F(c, in, iv, ie))));
     }
-    coll = c13 * coll;
-    dive = dive - coll;
-    dive = dive * dte * 0.5;
+    coll = AP(c13 * coll);
+    dive = AP(dive - coll);
+    dive = AP(AP(dive * dte) * 0.5);
     c->umean[ie - 1] = dive;
-    for (int in = 0; in < 3; in++) unkel[in] = unkel[in] - dive;
+    for (int in = 0; in < 3; in++) unkel[in] = AP(unkel[in] - dive);
     double fxme = 0.0, fyme = 0.0;
     for (int in = 0; in < 3; in++) {
       double fx = 0.0, fy = 0.0;
-      if (fabs(unkel[in]) != 0.0) { fx = ux * unkel[in]; fy = uy * unkel[in]; }
-      fxme = fxme + fx;
-      fyme = fyme + fy;
+      if (fabs(unkel[in]) != 0.0) { fx = AP(ux * unkel[in]); fy = AP(uy * unkel[in]); }
+      fxme = AP(fxme + fx);
+      fyme = AP(fyme + fy);
     }
-    fxme = c13 * fxme;
-    fyme = c13 * fyme;
+    fxme = AP(c13 * fxme);
+    fyme = AP(c13 * fyme);
     for (int in = 1; in <= 3; in++) {
-      double anx = GEOME(in, ie) * area, any = GEOME(in + 3, ie) * area;
+      double anx = AP(AP(GEOME(in, ie)) * area), any = AP(AP(GEOME(in + 3, ie)) * area);
       size_t k = NE(in, ie);
       c->rhs[k] = 0.0;
-      c->rhs[k] = c->rhs[k] + anx * fxme + any * fyme;
+      if (g_emu & ORA_EMU_FORM) continue; /* folded into the edge terms */
+      c->rhs[k] = AP(AP(c->rhs[k] + AP(anx * fxme)) + AP(any * fyme));
     }
   }
   if (g_stage) {
@@ -863,6 +1020,11 @@ static int ordinate(ora_ctx *c, int iv, double dte)
   for (size_t k = 0; k < 3 * (size_t)E; k++) c->delun[k] = 0.0;
   if (nt > 1) {
     int st = edges_threaded(c, iv, ux, uy);
+    if (st) return st;
+    goto edges_done;
+  }
+  if (g_emu & ORA_EMU_FORM) {
+    int st = edges_meanform(c, iv, ux, uy);
     if (st) return st;
     goto edges_done;
   }
@@ -882,28 +1044,29 @@ static int ordinate(ora_ctx *c, int iv, double dte)
     int upwind_left = scpr > 0.0;
     if (c->prm.mutant == ORA_MUTANT_DOWNWIND) upwind_left = !upwind_left;
     if (upwind_left) {
-      edgun[0] = F(c, inl1, iv, iel) - c->umean[iel - 1];
-      edgun[1] = F(c, inl2, iv, iel) - c->umean[iel - 1];
+      edgun[0] = AE(F(c, inl1, iv, iel) - c->umean[iel - 1]);
+      edgun[1] = AE(F(c, inl2, iv, iel) - c->umean[iel - 1]);
     } else {
-      edgun[0] = F(c, inr1, iv, ier) - c->umean[ier - 1];
-      edgun[1] = F(c, inr2, iv, ier) - c->umean[ier - 1];
+      edgun[0] = AE(F(c, inr1, iv, ier) - c->umean[ier - 1]);
+      edgun[1] = AE(F(c, inr2, iv, ier) - c->umean[ier - 1]);
     }
     double fx[2], fy[2], fn[2], rhsi[2];
     getflu2(edgun, ux, uy, fx, fy);
-    double alen = (1.0 / 6.0) * c->el[is - 1];
-    fn[0] = c->nx[is - 1] * fx[0] + c->ny[is - 1] * fy[0];
-    fn[1] = c->nx[is - 1] * fx[1] + c->ny[is - 1] * fy[1];
+    double alen = AE((1.0 / 6.0) * c->el[is - 1]);
+    const double snx = AE(c->nx[is - 1]), sny = AE(c->ny[is - 1]);
+    fn[0] = AE(AE(snx * fx[0]) + AE(sny * fy[0]));
+    fn[1] = AE(AE(snx * fx[1]) + AE(sny * fy[1]));
     edge_rhsi(fn, alen, rhsi);
-    c->rhs[NE(inr1, ier)] = c->rhs[NE(inr1, ier)] - rhsi[0];
-    c->rhs[NE(inr2, ier)] = c->rhs[NE(inr2, ier)] - rhsi[1];
-    c->rhs[NE(inl1, iel)] = c->rhs[NE(inl1, iel)] + rhsi[0];
-    c->rhs[NE(inl2, iel)] = c->rhs[NE(inl2, iel)] + rhsi[1];
+    c->rhs[NE(inr1, ier)] = AE(c->rhs[NE(inr1, ier)] - rhsi[0]);
+    c->rhs[NE(inr2, ier)] = AE(c->rhs[NE(inr2, ier)] - rhsi[1]);
+    c->rhs[NE(inl1, iel)] = AE(c->rhs[NE(inl1, iel)] + rhsi[0]);
+    c->rhs[NE(inl2, iel)] = AE(c->rhs[NE(inl2, iel)] + rhsi[1]);
   }
 edges_done:
   if (g_stage) memcpy(g_stage->rhs_edge, c->rhs, 3 * (size_t)E * sizeof(double));
   /* RHS *= DTE (src/advnce.f90:342-346) */
 #pragma omp parallel for schedule(static) if (nt > 1) num_threads(nt)
-  for (size_t k = 0; k < 3 * (size_t)E; k++) c->rhs[k] = dte * c->rhs[k];
+  for (size_t k = 0; k < 3 * (size_t)E; k++) c->rhs[k] = AI(dte * c->rhs[k]);
   /* GETINC (src/getinc.f90:22-43); the largest |increment| (a max, exact in
    * any order) */
   double res = 0.0;
@@ -912,11 +1075,11 @@ edges_done:
     for (int in = 1; in <= 3; in++) {
       size_t k = NE(in, ie);
       double nf = F(c, in, iv, ie);
-      double coll = c->mu[k] * (c->nfo[k] - nf);
-      double r1 = c->mmat[k] * c->rhs[k] + coll * dte;
+      double coll = AI(P(c->mu[k]) * AI(c->nfo[k] - nf));
+      double r1 = AI(AI(AI(c->mmat[k]) * c->rhs[k]) + AI(coll * dte));
       if (c->prm.mutant == ORA_MUTANT_RES_AFTER && c->inflow_node[k]) r1 = 0.0;
       if (fabs(r1) > res) res = fabs(r1);
-      c->delun[k] = c->delun[k] + c->mmat[k] * c->rhs[k] + coll * dte;
+      c->delun[k] = AI(AI(c->delun[k] + AI(AI(c->mmat[k]) * c->rhs[k])) + AI(coll * dte));
     }
   c->residual[iv - 1] = res;
   if (g_stage) memcpy(g_stage->delun, c->delun, 3 * (size_t)E * sizeof(double));
@@ -924,8 +1087,13 @@ edges_done:
 #pragma omp parallel for schedule(static) if (nt > 1) num_threads(nt)
   for (int ie = 1; ie <= E; ie++)
     for (int in = 1; in <= 3; in++) {
-      double v = F(c, in, iv, ie) + c->delun[NE(in, ie)];
+      double v;
+      if (g_emu & ORA_EMU_STORE_SR)
+        v = emu_store_sr(F(c, in, iv, ie), c->delun[NE(in, ie)], ie, iv, c->itime);
+      else
+        v = F(c, in, iv, ie) + c->delun[NE(in, ie)];
       if (v < 0.0) v = 0.0;
+      if (g_emu & (ORA_EMU_STORE_RN | ORA_EMU_STORE_SR)) v = (double)(float)v;
       F(c, in, iv, ie) = v;
     }
   /* INFLOW (src/inflow.f90:28-59) */
@@ -933,6 +1101,7 @@ edges_done:
   for (int ib = 1; ib <= c->B; ib++) if (BSIDO(4, ib) == 1) { have_inflow = 1; break; }
   if (have_inflow) {
     double nf = ora_inflow_value(c->cinf, c->prm.rgas, c->prm.m, ux, uy);
+    if (g_emu & (ORA_EMU_STORE_RN | ORA_EMU_STORE_SR)) nf = (double)(float)nf;
     if (c->prm.faithful_scans) {
       for (int ib = 1; ib <= c->B; ib++) {
         if (BSIDO(4, ib) != 1) continue;
@@ -977,10 +1146,13 @@ int ora_step(ora_ctx *c, int nsteps, ora_step_info *info)
     double dte = (dteb < c->dtec) ? dteb : c->dtec;
     if (c->prm.jacobi)
       memcpy(c->fsnap, c->f, 3 * (size_t)c->V * c->E * sizeof(double));
+    g_emu = c->emul;
+    if (g_emu & ORA_EMU_DTE) dte = (double)(float)dte;
     for (int iv = 1; iv <= c->V; iv++) {
       st = ordinate(c, iv, dte);
-      if (st) return st;
+      if (st) { g_emu = 0; return st; }
     }
+    g_emu = 0;
     double sumres = 0.0;
     for (int iv = 0; iv < c->V; iv++)
       if (fabs(c->residual[iv]) > sumres) sumres = fabs(c->residual[iv]);

@@ -89,6 +89,21 @@ int ora_step(ora_ctx *c, int nsteps, ora_step_info *info);
  * terms keep the serial order); used for the all-core CPU baseline. */
 int ora_set_threads(ora_ctx *c, int nthreads);
 
+/* FP32 emulation of stages of the step, for precision studies of the
+ * FP32-bulk form (the restatement itself is FP64; flags 0 restores it).
+ * Storing f rounds the current f once. */
+#define ORA_EMU_STORE_RN 1 /* f stored in FP32, rounded to nearest */
+#define ORA_EMU_STORE_SR 2 /* f stored in FP32 with a +-1/2 ulp dither (stochastic) */
+#define ORA_EMU_PARAMS 4   /* Maxwellian parameters and mu in FP32, FP32 exponential */
+#define ORA_EMU_ARITH 8    /* predictor, volume, edge, wall and increment arithmetic in FP32 */
+#define ORA_EMU_DTE 16     /* the time step rounded to FP32 */
+#define ORA_EMU_PRED 32    /* only the predictor and volume term (GETFLA, GETELC) in FP32 */
+#define ORA_EMU_EDGE 64    /* only the edge and wall fluxes (EDGFLXOPT, GETBOU) in FP32 */
+#define ORA_EMU_INC 128    /* only the increment (RHS*DTE, GETINC) in FP32 */
+#define ORA_EMU_FORM 256   /* difference-form predictor and the volume term folded
+                              into the edges (not the reference's order; studies) */
+void ora_set_emulation(ora_ctx *c, int flags);
+
 /* DIS* node fields (3,E) and point fields (P) from the most recent MACROS. */
 void ora_get_node_fields(const ora_ctx *c, double *disnd, double *disux,
                          double *disuy, double *disps, double *mu);

@@ -58,6 +58,8 @@ def lib():
         L.ora_step.restype = C.c_int
         L.ora_set_threads.argtypes = [C.c_void_p, C.c_int]
         L.ora_set_threads.restype = C.c_int
+        L.ora_set_emulation.argtypes = [C.c_void_p, C.c_int]
+        L.ora_set_emulation.restype = None
         L.ora_get_node_fields.argtypes = [C.c_void_p] + [_dp] * 5
         L.ora_get_point_fields.argtypes = [C.c_void_p] + [_dp] * 6
         L.ora_get_eta.argtypes = [C.c_void_p, _dp]
@@ -87,6 +89,8 @@ def lib():
 
 
 MUTANT_DOWNWIND, MUTANT_SWAP_NODES, MUTANT_GETMAC_EVERY, MUTANT_RES_AFTER = 1, 2, 3, 4
+EMU_STORE_RN, EMU_STORE_SR, EMU_PARAMS, EMU_ARITH, EMU_DTE = 1, 2, 4, 8, 16
+EMU_PRED, EMU_EDGE, EMU_INC, EMU_FORM = 32, 64, 128, 256
 
 
 def make_params(prm: dict, lookup: int = 1, jacobi: int = 0, faithful: int = 0,
@@ -145,6 +149,10 @@ class OracleRun:
         """Threads for step() (the same values bit for bit as one thread)."""
         if lib().ora_set_threads(self.h, int(n)) != 0:
             raise MemoryError("ora_set_threads: table allocation failed")
+
+    def set_emulation(self, flags: int):
+        """FP32 emulation of stages of the step (EMU_* flags; 0 = FP64)."""
+        lib().ora_set_emulation(self.h, int(flags))
 
     def get_f(self) -> np.ndarray:
         """f in the reference layout, shape (E, V, 3)."""
