@@ -201,43 +201,84 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
     cold[k] = s.pMd[k] * (feq - f[k]);
   }
   const T cs = cold[0] + cold[1] + cold[2];
-  // GETELC's predictor: U = DTE/2 (div(c f) - (1/3) sum MU (f_eq - f))
-  const T sx = s.gx[0] * f[0] + s.gx[1] * f[1] + s.gx[2] * f[2];
-  const T sy = s.gy[0] * f[0] + s.gy[1] * f[1] + s.gy[2] * f[2];
-  const T U = T(0.5) * ((ux * sx + uy * sy) * dte - c13 * cs);
+  T rhs[3], U;
+  if constexpr (sizeof(T) == 8) {
+    // GETELC's predictor: U = DTE/2 (div(c f) - (1/3) sum MU (f_eq - f))
+    const T sx = s.gx[0] * f[0] + s.gx[1] * f[1] + s.gx[2] * f[2];
+    const T sy = s.gy[0] * f[0] + s.gy[1] * f[1] + s.gy[2] * f[2];
+    U = T(0.5) * ((ux * sx + uy * sy) * dte - c13 * cs);
 #pragma unroll
-  for (int k = 0; k < 3; ++k) h[k] = f[k] - U;
-  store4(sf + 4 * slot, h[0], h[1], h[2], T(0));
-  __syncthreads();
-  if (!s.owned) return;
-  // volume term (src/getelc.f90:87-117): area * grad N_a . c (mean of f - U)
-  const T sm = c13 * (h[0] + h[1] + h[2]);
-  const T fx = ux * sm, fy = uy * sm;
-  T rhs[3];
+    for (int k = 0; k < 3; ++k) h[k] = f[k] - U;
+    store4(sf + 4 * slot, h[0], h[1], h[2], T(0));
+    __syncthreads();
+    if (!s.owned) return;
+    // volume term (src/getelc.f90:87-117): area * grad N_a . c (mean of f - U)
+    const T sm = c13 * (h[0] + h[1] + h[2]);
+    const T fx = ux * sm, fy = uy * sm;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) rhs[k] = s.gxa[k] * fx + s.gya[k] * fy;
-  // edges (src/edgflxopt.f90:72-125, src/getbou.f90:142-187).  Local edge k
-  // runs from node k to node k1 = (k+1)%3 with this element's outward
-  // normal n; its flux adds -(len/6) (c.n) M2 (g_k, g_k1) with
-  // M2 = [[2,1],[1,2]] and g = f - U of the upwind element.  The reference
-  // orients each side once (left element) and subtracts the left element's
-  // flux from the right element; with the right element's outward normal
-  // that is the same expression.  Upwind: own values when c.n > 0 (on a
-  // tie the right element's values, as in the reference), always on a
-  // boundary side; inflow and wall sides carry a zero (len/6) n here.
+    for (int k = 0; k < 3; ++k) rhs[k] = s.gxa[k] * fx + s.gya[k] * fy;
+    // edges (src/edgflxopt.f90:72-125, src/getbou.f90:142-187).  Local edge k
+    // runs from node k to node k1 = (k+1)%3 with this element's outward
+    // normal n; its flux adds -(len/6) (c.n) M2 (g_k, g_k1) with
+    // M2 = [[2,1],[1,2]] and g = f - U of the upwind element.  The reference
+    // orients each side once (left element) and subtracts the left element's
+    // flux from the right element; with the right element's outward normal
+    // that is the same expression.  Upwind: own values when c.n > 0 (on a
+    // tie the right element's values, as in the reference), always on a
+    // boundary side; inflow and wall sides carry a zero (len/6) n here.
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int k1 = (k + 1) % 3;
-    const int in = s.info[k];
-    // w = -(len/6) c.n from the edge's (len/6) n: c.n > 0 exactly when w < 0
-    const T w = -(ux * s.nx[k] + uy * s.ny[k]);
-    const bool own = (in & kUpdOwn) || w < T(0) || ((in & kUpdRight) && w == T(0));
-    // the neighbour's values are read whichever side is upwind (fixed LDS
-    // indices, no address arithmetic); the element's own are in registers
-    const T n1 = sf[s.nb1[k]], n2 = sf[s.nb2[k]];
-    const T g1 = own ? h[k] : n1, g2 = own ? h[k1] : n2;
-    rhs[k] += w * (T(2) * g1 + g2);
-    rhs[k1] += w * (g1 + T(2) * g2);
+    for (int k = 0; k < 3; ++k) {
+      const int k1 = (k + 1) % 3;
+      const int in = s.info[k];
+      // w = -(len/6) c.n from the edge's (len/6) n: c.n > 0 exactly when w < 0
+      const T w = -(ux * s.nx[k] + uy * s.ny[k]);
+      const bool own = (in & kUpdOwn) || w < T(0) || ((in & kUpdRight) && w == T(0));
+      // the neighbour's values are read whichever side is upwind (fixed LDS
+      // indices, no address arithmetic); the element's own are in registers
+      const T n1 = sf[s.nb1[k]], n2 = sf[s.nb2[k]];
+      const T g1 = own ? h[k] : n1, g2 = own ? h[k1] : n2;
+      rhs[k] += w * (T(2) * g1 + g2);
+      rhs[k1] += w * (g1 + T(2) * g2);
+    }
+  } else {
+    // FP32: the same operator in exact arithmetic, evaluated on differences
+    // of nearby values so that FP32 rounds the small terms, not f.
+    // Predictor: the gradients sum to zero, so div(c f) takes f - f_0.
+    const T d1 = f[1] - f[0], d2 = f[2] - f[0];
+    const T sx = s.gx[1] * d1 + s.gx[2] * d2;
+    const T sy = s.gy[1] * d1 + s.gy[2] * d2;
+    U = T(0.5) * ((ux * sx + uy * sy) * dte - c13 * cs);
+    // neighbours read raw f and U: each reader forms differences from its
+    // own values before anything is rounded at the size of f
+    store4(sf + 4 * slot, f[0], f[1], f[2], U);
+    __syncthreads();
+    if (!s.owned) return;
+    // area grad N_k is half the sum of len n over the element's edges at
+    // node k, so the volume term folds into the edges: each edge acts on
+    // (upwind g - the element's mean gbar of f - U), the mean about f_0 as
+    // gbar = f_0 + db - U with db the mean of (0, d1, d2)
+    const T db = c13 * (d1 + d2);
+    h[0] = -db;
+    h[1] = d1 - db;
+    h[2] = d2 - db;
+    const T gbar = (f[0] + db) - U;  // an inflow side's upwind values are 0
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rhs[k] = T(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int k1 = (k + 1) % 3;
+      const int in = s.info[k];
+      const T w = -(ux * s.nx[k] + uy * s.ny[k]);
+      const bool own = (in & kUpdOwn) || w < T(0) || ((in & kUpdRight) && w == T(0));
+      const T n1 = sf[s.nb1[k]], n2 = sf[s.nb2[k]], un = sf[s.nb1[k] | 3];
+      const T du = un - U;
+      const T e1 = ((n1 - f[0]) - db) - du, e2 = ((n2 - f[0]) - db) - du;
+      const bool inflow = (in & kUpdInflow) != 0;
+      const T g1 = own ? (inflow ? -gbar : h[k]) : e1;
+      const T g2 = own ? (inflow ? -gbar : h[k1]) : e2;
+      rhs[k] += w * (T(2) * g1 + g2);
+      rhs[k1] += w * (g1 + T(2) * g2);
+    }
   }
   if (s.wsl >= 0) {
     T* wp = a.wpart + ((size_t)s.wsl * a.V + v) * 8;
@@ -514,6 +555,17 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
       const T ux = a.vtab[v], uy = a.vtab[V + v];
       const T* fc = a.fcur + (size_t)v * a.ldf + 3 * (size_t)e;
       const T f[3] = {fc[0], fc[1], fc[2]};
+      // FP32: the edge terms act on (g - gbar), as in k_update's FP32 form
+      // (the element's own values as differences from f_0)
+      T hw[3] = {T(0), T(0), T(0)}, gbar = T(0);
+      if constexpr (sizeof(T) == 4) {
+        const T d1 = f[1] - f[0], d2 = f[2] - f[0];
+        const T db = (T(1) / T(3)) * (d1 + d2);
+        hw[0] = -db;
+        hw[1] = d1 - db;
+        hw[2] = d2 - db;
+        gbar = (f[0] + db) - U;
+      }
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         if ((info[k] & 3) != 3) continue;
@@ -550,6 +602,15 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
           const T mf = mw * (T)a.eta[a.wall_bside[ws]];
           g1 = (T(1) - alpha) * rf1 + alpha * mf;
           g2 = (T(1) - alpha) * rf2 + alpha * mf;
+        }
+        if constexpr (sizeof(T) == 4) {  // FP32 form: g - gbar
+          if (i1 < 0) {
+            g1 = hw[a1];
+            g2 = hw[a2];
+          } else {
+            g1 = g1 - gbar;
+            g2 = g2 - gbar;
+          }
         }
         T r1, r2;
         edge_rhsi(ux, uy, enx[k], eny[k], eal[k], g1, g2, r1, r2);
@@ -589,8 +650,8 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
-// k_moments: MACROS (src/macros.f90:80-164) and COMPMU (src/compmu.f90:
-// 299-347) per DG node, FP64 compensated sums over all ordinates.  Pass 1:
+// k_moments: MACROS (src/macros.f90:80-164) and COMPMU
+// (src/compmu.f90:36-97) per DG node, FP64 compensated sums over all ordinates.  Pass 1:
 // n, n*u, n*v; pass 2: pressure and mean relative speed about u.  Writes the
 // FP64 node fields and the next step's GTEQNF/COMPMU parameters in T.
 // ---------------------------------------------------------------------------
@@ -1515,10 +1576,13 @@ int upload_tables(Ctx& c) {
       eal[i] = (T)ed.alen;
       // update kernel: (len/6) times the outward normal (the right
       // element's is the side's normal negated), rounded once; zero on
-      // inflow and wall sides (no flux here).  The neighbour's nodes at
-      // this edge's nodes k and k+1.
+      // wall sides (k_wall's), and on inflow sides in FP64 (no flux); the
+      // FP32 form's edge terms carry the volume term's share, so an inflow
+      // side keeps its normal there and takes zero upwind values.  The
+      // neighbour's nodes at this edge's nodes k and k+1.
       const double sg = ed.sign > 0 ? 1.0 : -1.0;
-      const bool flux = ed.kind == kInterior || ed.kind == kOutflow;
+      const bool flux = ed.kind == kInterior || ed.kind == kOutflow ||
+                        (sizeof(T) == 4 && ed.kind == kInflow);
       unx[i] = flux ? (T)(sg * ed.alen * ed.nx) : T(0);
       uny[i] = flux ? (T)(sg * ed.alen * ed.ny) : T(0);
       if (ed.kind == kInterior) {
@@ -1526,7 +1590,7 @@ int upload_tables(Ctx& c) {
         uinfo[i] = (nbk << kUpdNbK) | (nbk1 << kUpdNbK1) | (ed.sign > 0 ? 0 : kUpdRight) |
                    (p.nbr_slot[3 * (size_t)e + k] << kUpdSlot);
       } else {
-        uinfo[i] = kUpdOwn;
+        uinfo[i] = kUpdOwn | (ed.kind == kInflow ? kUpdInflow : 0);
       }
     }
     emask[e] = p.inflow_mask[e];

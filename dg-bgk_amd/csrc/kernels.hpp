@@ -9,8 +9,9 @@
 //   eslot    int[3][E]  interior: neighbour's LDS slot; wall: wall-side slot
 //   enx/eny/ealen T[3][E] side normal and length/6 of each element edge
 //   uinfo/unx/uny        the same edges for the update kernel: (len/6)
-//                        times the outward normal (zero on inflow and wall
-//                        sides), neighbour slot and nodes (see kUpd*)
+//                        times the outward normal (zero on wall sides, and
+//                        on inflow sides in FP64), neighbour slot and nodes
+//                        (see kUpd*)
 //   np       T[5][3E]   per node: n*beta2/pi, beta2, u, v, mu (GTEQNF/COMPMU)
 //   vtab     T[4][V]    ordinate velocity (ADVNCE form), inflow value, and
 //                       the FP32 store dither's ordinate phase
@@ -40,6 +41,7 @@ constexpr int kUpdNbK = 0;      // bits 0-1: neighbour's local node at node k
 constexpr int kUpdNbK1 = 2;     // bits 2-3: neighbour's local node at node k1
 constexpr int kUpdRight = 1 << 4;  // this element is the side's right element
 constexpr int kUpdOwn = 1 << 5;    // boundary side: own values are always upwind
+constexpr int kUpdInflow = 1 << 6; // inflow side: upwind values are zero (FP32 form)
 constexpr int kUpdSlot = 8;     // bits 8+: neighbour's LDS slot
 
 template <typename T>

@@ -4,7 +4,7 @@
 // steady state (rho, u, v, T): Linear(5,W) + SiLU, (L-1) x [Linear(W,W) +
 // SiLU], Linear(W,4).  Weights and activations are BF16, accumulation and
 // master weights FP32, optimiser Adam.  The W x W layers (all but ~1% of the
-// work) run on v_mfma_f32_32x32x8_bf16 through one tiled GEMM kernel with
+// work) run on gfx950's v_mfma_f32_32x32x16_bf16 through one tiled GEMM kernel with
 // three epilogues: forward (bias + SiLU, keeps the pre-activation), backward
 // data (times SiLU' of the previous pre-activation) and backward weight
 // (split-K partials over the batch).  The 5-wide input and 4-wide output
@@ -25,7 +25,7 @@
 namespace dgbgk {
 namespace mlp {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef unsigned short bf16;
 
@@ -94,12 +94,14 @@ __global__ void __launch_bounds__(256) k_gemm(const bf16* __restrict__ A, int ld
       for (int j = 0; j < 8; ++j) Bs[(nq + j) * kLd + k] = h[j];
     }
     __syncthreads();
+    // gfx950's 32x32x16 BF16 MFMA: lane l supplies row (col) l % 32 of A
+    // (B) at k = 8 (l / 32) .. 8 (l / 32) + 7 of the 16-wide k step
 #pragma unroll
-    for (int kk = 0; kk < kKC; kk += 8) {
-      const int ko = kk + 4 * (lane >> 5);
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(&As[(wm + (lane & 31)) * kLd + ko]);
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(&Bs[(wn + (lane & 31)) * kLd + ko]);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    for (int kk = 0; kk < kKC; kk += 16) {
+      const int ko = kk + 8 * (lane >> 5);
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[(wm + (lane & 31)) * kLd + ko]);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[(wn + (lane & 31)) * kLd + ko]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
     __syncthreads();
   }

@@ -121,15 +121,25 @@ def cylinder_flow(target_elems: int = 100_000, rorder: int = 16, torder: int = 3
     return Case("cylinder_flow", m, eta, w, torder, prm, nsteps=2000)
 
 
+def particle_mesh(target_elems: int = 1_000_000, far_types=(1, 2, 2, 2),
+                  h0: float | None = None) -> meshgen.Mesh:
+    """200D x 100D box, particle of diameter D = 1 at (60, 50), graded
+    Delaunay; h0 the spacing on the particle (default 0.02, or hmax/4 on
+    coarse meshes).  far_types: (left, right, bottom, top) box side types."""
+    hmax = 0.215 * math.sqrt(1_000_000 / target_elems)
+    return meshgen.body_in_box(200.0, 100.0, 60.0, 50.0, 1.0,
+                               h0=min(0.02, hmax / 4) if h0 is None else h0,
+                               growth=0.1, hmax=hmax, seed=5, wall_type=4, tw=1,
+                               alpha100=100, far_types=tuple(far_types),
+                               name=f"particle_{target_elems}")
+
+
 def particle_drag(target_elems: int = 1_000_000, rorder: int = 32, torder: int = 64,
-            mach: float = 0.3, kn: float = 1.0) -> Case:
+            mach: float = 0.3, kn: float = 1.0, far_types=(1, 2, 2, 2),
+            h0: float | None = None) -> Case:
     """Particle of diameter D in a 200D x 100D box, ~1M graded triangles,
     2048 ordinates, Mach 0.3, Kn 1."""
-    hmax = 0.215 * math.sqrt(1_000_000 / target_elems)
-    m = meshgen.body_in_box(200.0, 100.0, 60.0, 50.0, 1.0, h0=min(0.02, hmax / 4),
-                            growth=0.1, hmax=hmax, seed=5, wall_type=4, tw=1,
-                            alpha100=100, far_types=(1, 2, 2, 2),
-                            name=f"particle_{target_elems}")
+    m = particle_mesh(target_elems, far_types, h0)
     u0 = mach_to_u(mach)
     eta, w = gauss_legendre_speeds(rorder)
     prm = base_params(u0=u0, kn=kn, length=1.0, rv=rv_for(u0), alpha=1.0)

@@ -57,15 +57,43 @@ class Sample:
     kn: float
 
 
-def draw(n: int, seed: int) -> list[Sample]:
+@dataclass(frozen=True)
+class Geometry:
+    """A body-in-box geometry of the warm start: the box (for the feature
+    scaling), the wall-distance scale, the spacing on the body, and the
+    (Mach, Kn) ranges its training states are drawn from."""
+    box: tuple
+    wd_scale: float
+    mach: tuple
+    kn: tuple
+    h0: float | None = None
+
+
+GEOMETRIES = {
+    # the cylinder flow's box: 20D x 10D, cylinder at (5D, 5D)
+    "cylinder": Geometry(box=(20.0, 10.0), wd_scale=10.0, mach=(0.1, 1.5), kn=(KN_MIN, 1.0)),
+    # the particle-drag box: 200D x 100D, particle at (60D, 50D); Mach and
+    # Kn about the large case's (0.3, 1)
+    "particle": Geometry(box=(200.0, 100.0), wd_scale=100.0, mach=(0.1, 0.8), kn=(0.2, 2.0),
+                         h0=0.1),
+}
+
+
+def draw(n: int, seed: int, geometry: str = "cylinder") -> list[Sample]:
+    g = GEOMETRIES[geometry]
     rng = np.random.default_rng(seed)
-    mach = rng.uniform(0.1, 1.5, n)
-    kn = 10 ** rng.uniform(math.log10(KN_MIN), 0.0, n)
+    mach = rng.uniform(g.mach[0], g.mach[1], n)
+    kn = 10 ** rng.uniform(math.log10(g.kn[0]), math.log10(g.kn[1]), n)
     return [Sample(float(m), float(k)) for m, k in zip(mach, kn)]
 
 
-def mesh_for(elems: int) -> meshgen.Mesh:
-    return cases.cylinder_mesh(elems, far_types=FAR_FIELD)
+def mesh_for(elems: int, geometry: str = "cylinder", h0: float | None = None) -> meshgen.Mesh:
+    """The geometry's mesh of about `elems` triangles, freestream held on all
+    four box sides."""
+    if geometry == "cylinder":
+        return cases.cylinder_mesh(elems, far_types=FAR_FIELD)
+    g = GEOMETRIES[geometry]
+    return cases.particle_mesh(elems, far_types=FAR_FIELD, h0=g.h0 if h0 is None else h0)
 
 
 def cylinder_case(mesh, s: Sample, rorder: int, torder: int) -> cases.Case:
@@ -75,10 +103,11 @@ def cylinder_case(mesh, s: Sample, rorder: int, torder: int) -> cases.Case:
     return cases.Case(f"cyl_M{s.mach:.3f}_Kn{s.kn:.4f}", mesh, eta, w, torder, prm)
 
 
-def features(mesh, wd, s: Sample) -> np.ndarray:
+def features(mesh, wd, s: Sample, geometry: str = "cylinder") -> np.ndarray:
+    g = GEOMETRIES[geometry]
     n = mesh.npoin
-    return np.stack([mesh.coord[:, 0] / 20.0, mesh.coord[:, 1] / 10.0,
-                     np.log1p(wd) / math.log1p(10.0), np.full(n, s.mach),
+    return np.stack([mesh.coord[:, 0] / g.box[0], mesh.coord[:, 1] / g.box[1],
+                     np.log1p(wd) / math.log1p(g.wd_scale), np.full(n, s.mach),
                      np.full(n, math.log10(s.kn))], axis=1).astype(np.float32)
 
 
@@ -195,12 +224,12 @@ def _save(path, d):
 
 
 def generate(elems, n, seed, rorder, torder, tol, max_steps, data, time_budget=None,
-             log=print):
+             log=print, geometry="cylinder"):
     """Steady FP64 states of n drawn cases, appended to the npz at `data`
     (resumable: cases already there are skipped)."""
-    mesh = mesh_for(elems)
+    mesh = mesh_for(elems, geometry)
     wd = meshgen.wall_distance(mesh)
-    samples = draw(n, seed)
+    samples = draw(n, seed, geometry)
     d = _load(data) or dict(mach=np.zeros(0), kn=np.zeros(0), steps=np.zeros(0, np.int64),
                             wall_s=np.zeros(0), drag=np.zeros(0), converged=np.zeros(0, bool),
                             y=np.zeros((0, mesh.npoin, 4), np.float32))
@@ -228,14 +257,14 @@ def generate(elems, n, seed, rorder, torder, tol, max_steps, data, time_budget=N
 
 
 def train(elems, data, params, steps, width=256, hidden=6, batch=65536, lr=1e-3, seed=0,
-          log=print):
+          log=print, geometry="cylinder"):
     from .mlp import MLP
-    mesh = mesh_for(elems)
+    mesh = mesh_for(elems, geometry)
     wd = meshgen.wall_distance(mesh)
     d = _load(data)
     keep = d["converged"]
     samples = [Sample(m, k) for m, k in zip(d["mach"][keep], d["kn"][keep])]
-    X = np.concatenate([features(mesh, wd, s) for s in samples])
+    X = np.concatenate([features(mesh, wd, s, geometry) for s in samples])
     Y = np.concatenate([net_targets(y, s) for y, s in zip(d["y"][keep], samples)])
     log(f"  training set: {int(keep.sum())} steady states, {len(X)} points")
     m = MLP(fin=X.shape[1], width=width, hidden=hidden, fout=Y.shape[1], batch=batch)
@@ -255,14 +284,16 @@ def train(elems, data, params, steps, width=256, hidden=6, batch=65536, lr=1e-3,
     train_s = time.perf_counter() - t0
     np.savez(params, params=m.get_params(), width=width, hidden=hidden,
              losses=np.asarray(losses, np.float32), train_s=train_s, npoints=len(X),
-             ncases=int(keep.sum()))
+             ncases=int(keep.sum()), geometry=geometry)
     return m, np.asarray(losses), train_s
 
 
-def load_model(params):
+def load_model(params, batch: int = 65536):
+    """The trained network, for inference in batches of `batch` points (a
+    multiple of 64: each batch is one pass of the GEMM kernels)."""
     from .mlp import MLP
     with np.load(params, allow_pickle=False) as z:
-        m = MLP(fin=5, width=int(z["width"]), hidden=int(z["hidden"]), fout=4, batch=64)
+        m = MLP(fin=5, width=int(z["width"]), hidden=int(z["hidden"]), fout=4, batch=batch)
         m.set_params(z["params"])
     return m
 
@@ -272,19 +303,19 @@ RUNS = {"warm64": (_lib.DGBGK_FP64, True), "cold32": (_lib.DGBGK_FP32, False),
 
 
 def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print,
-             runs=("warm64", "cold32", "warm32"), ref_tol=None):
+             runs=("warm64", "cold32", "warm32"), ref_tol=None, geometry="cylinder"):
     """Per held-out case: the FP64 run from the freestream (cold64) and the
     requested others (FP64/FP32-bulk, from the freestream or from the
     network's state), all judged by cold64's step-1 change.  With ref_tol,
     cold64 goes on to change < ref_tol x its step-1 change, and every run's
     drag is also compared with that converged drag (drag_rel_ref)."""
     m = load_model(params)
-    mesh = mesh_for(elems)
+    mesh = mesh_for(elems, geometry)
     wd = meshgen.wall_distance(mesh)
     out = []
-    for s in draw(n, seed):
+    for s in draw(n, seed, geometry):
         case = cylinder_case(mesh, s, rorder, torder)
-        pred = untarget(m.forward(features(mesh, wd, s)), s)
+        pred = untarget(m.forward(features(mesh, wd, s, geometry)), s)
         cold64 = run_to_steady(case, tol, max_steps, _lib.DGBGK_FP64, log=log, ref_tol=ref_tol)
         norm = cold64["first_change"]
         done = dict(cold64=cold64)
@@ -362,6 +393,7 @@ def main(argv=None):
     ap.add_argument("--data", default="gpurun_out/warmstart_train.npz")
     ap.add_argument("--params", default="gpurun_out/warmstart_params.npz")
     ap.add_argument("--out", default="gpurun_out/warmstart_eval.json")
+    ap.add_argument("--geometry", default="cylinder", choices=sorted(GEOMETRIES))
     ap.add_argument("--runs", default="warm64,cold32,warm32",
                     help="runs besides the FP64 freestream run, of warm64, cold32, warm32")
     a = ap.parse_args(argv)
@@ -369,19 +401,19 @@ def main(argv=None):
     os.makedirs(os.path.dirname(os.path.abspath(a.data)), exist_ok=True)
     if a.cmd == "generate":
         d = generate(a.elems, a.n, a.seed, a.rorder, a.torder, a.tol, a.max_steps, a.data,
-                     a.time_budget or None, log)
+                     a.time_budget or None, log, a.geometry)
         log(json.dumps(dict(cases=len(d["mach"]), converged=int(d["converged"].sum()),
                             median_steps=float(np.median(d["steps"])),
                             total_wall_s=float(d["wall_s"].sum()))))
     elif a.cmd == "train":
         _, losses, train_s = train(a.elems, a.data, a.params, a.train_steps, batch=a.batch,
-                                   lr=a.lr, log=log)
+                                   lr=a.lr, log=log, geometry=a.geometry)
         log(json.dumps(dict(train_s=train_s, loss_first=float(losses[:20].mean()),
                             loss_last=float(losses[-50:].mean()))))
     else:
         res = evaluate(a.elems, a.params, a.n, a.seed, a.rorder, a.torder, a.tol,
                        a.max_steps, log, runs=tuple(x for x in a.runs.split(",") if x),
-                       ref_tol=a.ref_tol or None)
+                       ref_tol=a.ref_tol or None, geometry=a.geometry)
         summ = summarise(res)
         with open(a.out, "w") as fh:
             json.dump(dict(elems=a.elems, ordinates=a.rorder * a.torder, tol=a.tol,

@@ -894,9 +894,12 @@ static int edges_meanform(ora_ctx *c, int iv, double ux, double uy)
   if (!c->elem_sides && ora_set_threads(c, 2)) return -100;
   c->nthreads = 1;
   for (int ie = 1; ie <= E; ie++) {
-    const double gb = AE(c13 * AE(AE(AE(F(c, 1, iv, ie) - c->umean[ie - 1]) +
-                                     AE(F(c, 2, iv, ie) - c->umean[ie - 1])) +
-                                  AE(F(c, 3, iv, ie) - c->umean[ie - 1])));
+    /* differences from the element's first node: r = f_1, dl_k = f_k - r,
+     * the mean of g = f - U is gb = r + db - U with db the mean of dl */
+    const double r = F(c, 1, iv, ie), uo = c->umean[ie - 1];
+    const double dl[3] = {0.0, AE(F(c, 2, iv, ie) - r), AE(F(c, 3, iv, ie) - r)};
+    const double db = AE(c13 * AE(dl[1] + dl[2]));
+    const double gb = AE(AE(r + db) - uo);
     for (int j = 0; j < 3; j++) {
       int code = c->elem_sides[3 * (size_t)(ie - 1) + j];
       if (code == 0) continue;
@@ -904,18 +907,32 @@ static int edges_meanform(ora_ctx *c, int iv, double ux, double uy)
       int iel = ISIDE(3, is), ier = ISIDE(4, is);
       int inl1 = ISIDE(5, is), inl2 = ISIDE(6, is);
       int inr1 = ISIDE(7, is), inr2 = ISIDE(8, is);
-      double g[2];
+      /* h = upwind g - gb, formed from differences of nearby values */
+      const int mine1 = code > 0 ? inl1 : inr1, mine2 = code > 0 ? inl2 : inr2;
+      double h[2];
+      int own = 0;
       if (ier == 0) {
+        double g[2];
         int st = getbou_values(c, is, iv, ux, uy, g);
         if (st) return st;
-      } else if (ux * c->nx[is - 1] + uy * c->ny[is - 1] > 0.0) {
-        g[0] = AE(F(c, inl1, iv, iel) - c->umean[iel - 1]);
-        g[1] = AE(F(c, inl2, iv, iel) - c->umean[iel - 1]);
+        int isb = c->bou_of_side[is - 1];
+        const double tst = c->cxa[iv - 1] * c->rsido[3 * (isb - 1)] +
+                           c->cya[iv - 1] * c->rsido[3 * (isb - 1) + 1];
+        const int index = c->bsido[6 * (isb - 1) + 3];
+        if (index != 1 && (index == 2 || tst >= 0.0)) own = 1;
+        else { h[0] = AE(g[0] - gb); h[1] = AE(g[1] - gb); }
       } else {
-        g[0] = AE(F(c, inr1, iv, ier) - c->umean[ier - 1]);
-        g[1] = AE(F(c, inr2, iv, ier) - c->umean[ier - 1]);
+        const int left_up = ux * c->nx[is - 1] + uy * c->ny[is - 1] > 0.0;
+        if (left_up == (code > 0)) own = 1;
+        else {
+          const int nb = code > 0 ? ier : iel;
+          const int n1 = code > 0 ? inr1 : inl1, n2 = code > 0 ? inr2 : inl2;
+          const double du = AE(c->umean[nb - 1] - uo);
+          h[0] = AE(AE(AE(F(c, n1, iv, nb) - r) - db) - du);
+          h[1] = AE(AE(AE(F(c, n2, iv, nb) - r) - db) - du);
+        }
       }
-      double h[2] = {AE(g[0] - gb), AE(g[1] - gb)};
+      if (own) { h[0] = AE(dl[mine1 - 1] - db); h[1] = AE(dl[mine2 - 1] - db); }
       double fx[2], fy[2], fn[2], rhsi[2];
       getflu2(h, ux, uy, fx, fy);
       const double snx = ier == 0 ? AE(c->rsido[3 * (c->bou_of_side[is - 1] - 1)]) : AE(c->nx[is - 1]);
@@ -975,11 +992,11 @@ static int ordinate(ora_ctx *c, int iv, double dte)
     double dive = 0.0, coll = 0.0;
     for (int in = 1; in <= 3; in++) unkel[in - 1] = F(c, in, iv, ie);
     if (g_emu & ORA_EMU_FORM) { /* difference form: the gradients sum to zero */
-      const size_t k0 = NE(1, ie), k1 = NE(2, ie), k2 = NE(3, ie);
-      dive = AP(AP(AP(AP(GEOME(2, ie)) * AP(c->fluxp[k1] - c->fluxp[k0])) +
-                   AP(AP(GEOME(3, ie)) * AP(c->fluxp[k2] - c->fluxp[k0]))) +
-                AP(AP(AP(GEOME(5, ie)) * AP(c->fluyp[k1] - c->fluyp[k0])) +
-                   AP(AP(GEOME(6, ie)) * AP(c->fluyp[k2] - c->fluyp[k0]))));
+      const double d2 = AP(F(c, 2, iv, ie) - F(c, 1, iv, ie));
+      const double d3 = AP(F(c, 3, iv, ie) - F(c, 1, iv, ie));
+      const double sx = AP(AP(AP(GEOME(2, ie)) * d2) + AP(AP(GEOME(3, ie)) * d3));
+      const double sy = AP(AP(AP(GEOME(5, ie)) * d2) + AP(AP(GEOME(6, ie)) * d3));
+      dive = AP(AP(ux * sx) + AP(uy * sy));
     } else
     for (int in = 1; in <= 3; in++) {
       size_t k = NE(in, ie);

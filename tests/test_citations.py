@@ -43,5 +43,9 @@ def test_cited_lines_exist():
                     count += 1
                     if min(a) < 1 or max(a) > nlines[path] or a != sorted(a):
                         bad.append(f"{f}:{ln}: {path}:{part} (file has {nlines[path]} lines)")
+        # a citation broken over two lines escapes the check above
+        for ln, line in enumerate(text.splitlines(), 1):
+            if re.search(r"\.f90:\s*$", line):
+                bad.append(f"{f}:{ln}: citation broken over a line end")
     assert count > 100
     assert not bad, "\n".join(bad)

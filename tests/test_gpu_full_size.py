@@ -73,12 +73,17 @@ def test_cylinder_full_size_two_steps_against_oracle(cylinder):
     o.set_threads(THREADS)
     ro = o.step(2)
     g, onf, opf = o.get_f(), o.node_fields(), o.point_fields()
+    # the library's MU is COMPMU of the current f (the next step's); the
+    # oracle computes it the same way at the top of its next step
+    o.prepare_step()
+    omu = o.node_fields()["mu"]
     o.close()
+    assert rel_l2(nf["mu"], omu) < 1e-10
     err = rel_l2(f, g)
     del f, g
     _say(f"cylinder 2 steps: rel L2(f) = {err:.3e}")
     assert err < 1e-10
-    for k in ("disnd", "disps", "mu"):
+    for k in ("disnd", "disps"):
         assert rel_l2(nf[k], onf[k]) < 1e-10, k
     assert rel_l2_vec(nf["disux"], nf["disuy"], onf["disux"], onf["disuy"]) < 1e-10
     for k in ("nd", "rho", "ps", "temp"):
@@ -108,7 +113,8 @@ def test_cylinder_full_size_2000_steps_both_forms(cylinder):
     drag = abs(out["FP32-bulk"][1]["drag"] - out["FP64"][1]["drag"]) / abs(out["FP64"][1]["drag"])
     _say("cylinder 2000 steps, FP32-bulk vs FP64: " +
          ", ".join(f"{k} {v:.2e}" for k, v in errs.items()) + f", drag {drag:.2e}")
-    assert max(errs.values()) < 1e-4 and drag < 1e-4
+    # measured: rho 8.7e-7, p 1.6e-6, T 8.0e-7, (u, v) 1.2e-6, drag 6.6e-7
+    assert max(errs.values()) < 3.5e-6 and drag < 1.5e-6
 
 
 def test_particle_full_size_20_steps_both_forms(particle):
