@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1159,6 +1160,37 @@ __global__ void __launch_bounds__(256) k_fstats(const T* __restrict__ f, int E, 
   }
 }
 
+// Combines n per-block (min, max, non-finite count) triples of k_fstats
+// into one triple (one block): min and max are exact in any order, and the
+// counts are integers summed exactly in double.
+__global__ void __launch_bounds__(256) k_fstats_combine(const double* __restrict__ part, int n,
+                                                        double* __restrict__ out) {
+  __shared__ double sh[8];
+  double lo = INFINITY, hi = -INFINITY, bad = 0.0;
+  for (int b = threadIdx.x; b < n; b += blockDim.x) {
+    lo = fmin(lo, part[3 * (size_t)b]);
+    hi = fmax(hi, part[3 * (size_t)b + 1]);
+    bad += part[3 * (size_t)b + 2];
+  }
+  const double mlo = -block_max(-lo, sh);
+  const double mhi = block_max(hi, sh);
+  const double nb = block_sum(bad, sh);
+  if (threadIdx.x == 0) {
+    out[0] = mlo;
+    out[1] = mhi;
+    out[2] = nb;
+  }
+}
+
+// scratch: the allocation-free statistics entry point's work array
+// (dgbgk_f_stats_noalloc): k_fstats's per-block triples, then the combined
+// triple.  Each device holds its own copy for the life of the process; host
+// code reaches it only through the address hipGetSymbolAddress returns on
+// that device.  Room for kFstatsMaxBlocks blocks (16 per ordinate row at
+// up to 4096 ordinates): 1.5 MiB.
+constexpr int kFstatsMaxBlocks = 16 * 4096;
+static __device__ __align__(256) double scratch[3 * kFstatsMaxBlocks + 4];
+
 // Layout conversions between the reference's DISNF(3,V,E) (double, input
 // element numbering) and the device's velocity-major T[V][ldf] (storage
 // order; eorder maps a storage position to its input element, or is null
@@ -2293,6 +2325,61 @@ int dgbgk_f_stats(dgbgk_ctx* h, double* fmin, double* fmax, long long* nonfinite
   if (fmin) *fmin = lo;
   if (fmax) *fmax = hi;
   if (nonfinite) *nonfinite = (long long)bad;
+  return kOk;
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};  // scratch's address on each device
+
+// The scratch array's device address on the current device, looked up once
+// per device with that device current.
+int fstats_scratch_ptr(double** out) {
+  int dev = 0;
+  HIPCK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(kErrArg, "device id out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    HIPCK(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = (double*)ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return kOk;
+}
+}  // namespace
+
+int dgbgk_f_stats_noalloc(dgbgk_ctx* h, double* fmin, double* fmax, long long* nonfinite) {
+  if (!h) return fail(kErrArg, "null context");
+  Ctx& c = h->c;
+  const size_t N3 = 3 * (size_t)c.p.E;
+  const unsigned gx = (unsigned)std::min<size_t>(16, (N3 + 255) / 256);
+  const size_t nb = (size_t)gx * c.p.V;  // <= 16 * 4096 = kFstatsMaxBlocks
+  if (c.p.V > kFstatsMaxBlocks / 16 || nb > (size_t)kFstatsMaxBlocks)
+    return fail(kErrArg, "dgbgk_f_stats_noalloc: " + std::to_string(c.p.V) +
+                             " ordinates exceed its scratch (at most 4096)");
+  double* sp = nullptr;  // this device's copy of scratch
+  int st = fstats_scratch_ptr(&sp);
+  if (st) return st;
+  double* part = sp;
+  double* res = sp + 3 * (size_t)kFstatsMaxBlocks;
+  dim3 g(gx, c.p.V);
+  if (c.prec)
+    hipLaunchKernelGGL(k_fstats<float>, g, dim3(256), 0, c.stream, (const float*)c.f[c.cur],
+                       c.p.E, (long long)c.ldf, part);
+  else
+    hipLaunchKernelGGL(k_fstats<double>, g, dim3(256), 0, c.stream, (const double*)c.f[c.cur],
+                       c.p.E, (long long)c.ldf, part);
+  HIPCK(hipGetLastError());
+  hipLaunchKernelGGL(k_fstats_combine, dim3(1), dim3(256), 0, c.stream, part, (int)nb, res);
+  HIPCK(hipGetLastError());
+  double r[3];
+  HIPCK(hipMemcpyAsync(r, res, sizeof(r), hipMemcpyDeviceToHost, c.stream));
+  HIPCK(hipStreamSynchronize(c.stream));
+  if (fmin) *fmin = r[0];
+  if (fmax) *fmax = r[1];
+  if (nonfinite) *nonfinite = (long long)r[2];
   return kOk;
 }
 

@@ -32,7 +32,8 @@ EXPORTS = [
     "dgbgk_forces", "dgbgk_wall_eta", "dgbgk_sizes", "dgbgk_setup_tables",
     "dgbgk_setup_create", "dgbgk_setup_sizes", "dgbgk_setup_tables_host",
     "dgbgk_setup_destroy", "dgbgk_timing", "dgbgk_kernel_time", "dgbgk_sync",
-    "dgbgk_dof_bytes", "dgbgk_launch_config", "dgbgk_f_stats", "dgbgk_mlp_create",
+    "dgbgk_dof_bytes", "dgbgk_launch_config", "dgbgk_f_stats", "dgbgk_f_stats_noalloc",
+    "dgbgk_mlp_create",
     "dgbgk_mlp_destroy", "dgbgk_mlp_last_error",
     "dgbgk_mlp_nparams", "dgbgk_mlp_set_params", "dgbgk_mlp_get_params", "dgbgk_mlp_forward",
     "dgbgk_mlp_loss_grad", "dgbgk_mlp_set_data", "dgbgk_mlp_train",
@@ -114,6 +115,7 @@ def lib():
     L.dgbgk_dof_bytes.argtypes = [vp, C.POINTER(C.c_int)]
     L.dgbgk_f_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                 C.POINTER(C.c_longlong)]
+    L.dgbgk_f_stats_noalloc.argtypes = L.dgbgk_f_stats.argtypes
     L.dgbgk_launch_config.argtypes = [vp] + [C.POINTER(C.c_int)] * 6
     fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     ofp = _opt(fp)

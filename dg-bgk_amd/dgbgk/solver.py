@@ -193,6 +193,13 @@ class Solver:
         check(lib().dgbgk_f_stats(self.h, C.byref(lo), C.byref(hi), C.byref(bad)))
         return {"min": lo.value, "max": hi.value, "nonfinite": bad.value}
 
+    def f_stats_noalloc(self) -> dict:
+        """f_stats through the allocation-free entry point (fixed per-GPU
+        scratch; at most 4096 ordinates)."""
+        lo, hi, bad = C.c_double(), C.c_double(), C.c_longlong()
+        check(lib().dgbgk_f_stats_noalloc(self.h, C.byref(lo), C.byref(hi), C.byref(bad)))
+        return {"min": lo.value, "max": hi.value, "nonfinite": bad.value}
+
     def bytes_per_value(self) -> int:
         b = C.c_int()
         check(lib().dgbgk_dof_bytes(self.h, C.byref(b)))

@@ -131,6 +131,15 @@ int dgbgk_wall_eta(dgbgk_ctx *ctx, double *eta);
  * host copy of f). */
 int dgbgk_f_stats(dgbgk_ctx *ctx, double *fmin, double *fmax, long long *nonfinite);
 
+/* dgbgk_f_stats without any allocation (no counterpart in the reference;
+ * for callers that cannot allocate device memory: tight serving or stepping
+ * loops, code that forbids hipMalloc).  Same values as dgbgk_f_stats.  Its
+ * scratch is a fixed 1.5 MiB device array per GPU, held until the process
+ * ends, so at most 4096 ordinates (DGBGK_ERR_ARG beyond).  Calls that
+ * overlap on one GPU share that array and race; one call at a time per
+ * GPU. */
+int dgbgk_f_stats_noalloc(dgbgk_ctx *ctx, double *fmin, double *fmax, long long *nonfinite);
+
 /* Sizes: NSIDE, VNPNT, NELEM, NPOIN, NBOUN, steps taken. */
 int dgbgk_sizes(dgbgk_ctx *ctx, int *nside, int *nvel, int *nelem, int *npoin,
                 int *nboun, int *itime);
