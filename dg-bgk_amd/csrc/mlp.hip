@@ -45,6 +45,10 @@ __device__ __forceinline__ float dsilu(float z) {
 enum Epi { kFwd = 0, kBwdData = 1, kPartial = 2 };
 
 constexpr int kTile = 64, kKC = 32, kPad = 8, kLd = kKC + kPad;
+// rows per block of the row-split reductions (bias and small-layer weight
+// gradients): short enough that a 65,536-row batch gives 512 blocks per
+// output column set, so the loads of many blocks overlap
+constexpr int kRowChunk = 128;
 
 // C[M][N] = sum_k A(m,k) B(k,n), 64x64 per block, 32x32 per wave.
 // A(m,k) = kAT ? A[k*lda + m] : A[m*lda + k]
@@ -369,7 +373,7 @@ int forward(dgbgk_mlp* h, int rows, bool with_loss) {
 int backward(dgbgk_mlp* h, int rows) {
   const int W = h->width, L = h->nlayers - 1, H = h->hidden;
   const dim3 b256(256);
-  const int rchunk = 1024, nch = (rows + rchunk - 1) / rchunk;
+  const int rchunk = kRowChunk, nch = (rows + rchunk - 1) / rchunk;
   // output layer: weights and bias
   {
     const int n = h->fout * (W + 1);
@@ -485,7 +489,7 @@ int dgbgk_mlp_create(int fin, int width, int hidden, int fout, int batch, dgbgk_
   st |= dalloc(h, &h->Yb, (size_t)batch * fout);
   st |= dalloc(h, &h->Yout, (size_t)batch * fout);
   st |= dalloc(h, &h->dY, (size_t)batch * fout);
-  const int nch = (batch + 1023) / 1024;
+  const int nch = (batch + kRowChunk - 1) / kRowChunk;
   h->part_n = std::max({(size_t)64 * width * width, (size_t)nch * width * (fin + 1),
                         (size_t)nch * fout * (width + 1), (size_t)nch * width});
   st |= dalloc(h, &h->part, h->part_n);

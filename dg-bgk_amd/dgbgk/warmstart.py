@@ -353,6 +353,52 @@ def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print,
     return out
 
 
+def time_to_residual(case, params, sample: Sample, geometry: str = "cylinder",
+                     tol: float = 1e-6, max_steps: int = 2_000_000, ref_tol: float | None = None,
+                     log=None, chunk: int = 1000):
+    """Steps and wall time to a step change of tol times the plain run's
+    step-1 change: FP64 from the freestream (plain) against FP32-bulk from
+    the Maxwellian of the network's state (learned), both judged by the
+    plain run's step-1 change.  The learned run's wall time includes the
+    network's inference.  With ref_tol the plain run goes on to ref_tol for
+    a converged reference drag."""
+    mesh = case.mesh
+    plain = run_to_steady(case, tol, max_steps, _lib.DGBGK_FP64, log=log, ref_tol=ref_tol,
+                          chunk=chunk)
+    if log:
+        log(f"#   plain FP64: {plain['steps']} steps, {plain['wall_s']:.1f} s")
+    t0 = time.perf_counter()
+    m = load_model(params)
+    pred = untarget(m.forward(features(mesh, meshgen.wall_distance(mesh), sample, geometry)),
+                    sample)
+    m.close()
+    predict_s = time.perf_counter() - t0
+    learned = run_to_steady(case, tol, max_steps, _lib.DGBGK_FP32, pred, plain["first_change"],
+                            log=log, chunk=chunk)
+    if log:
+        log(f"#   learned FP32-bulk: {learned['steps']} steps, {learned['wall_s']:.1f} s "
+            f"(+ network {predict_s:.2f} s)")
+    wl = learned["wall_s"] + predict_s
+    out = {"tol": tol,
+           "plain_fp64": {"steps": int(plain["steps"]), "wall_s": plain["wall_s"],
+                          "converged": bool(plain["converged"]), "drag": plain["drag"]},
+           "learned_fp32": {"steps": int(learned["steps"]), "wall_s": wl, "network_s": predict_s,
+                            "converged": bool(learned["converged"]), "drag": learned["drag"]},
+           "speedup_wall": plain["wall_s"] / wl,
+           "speedup_steps": plain["steps"] / max(learned["steps"], 1),
+           "drag_rel": abs(learned["drag"] - plain["drag"]) / abs(plain["drag"]),
+           "pred_rho_rel_l2": float(np.linalg.norm(pred[0] - plain["fields"]["rho"]) /
+                                    np.linalg.norm(plain["fields"]["rho"]))}
+    if ref_tol is not None and plain["ref_drag"] is not None:
+        ref = plain["ref_drag"]
+        out["ref_tol"] = ref_tol
+        out["ref_drag"] = ref
+        out["ref_steps"] = plain["ref_steps"]
+        out["plain_fp64"]["drag_rel_ref"] = abs(plain["drag"] - ref) / abs(ref)
+        out["learned_fp32"]["drag_rel_ref"] = abs(learned["drag"] - ref) / abs(ref)
+    return out
+
+
 def summarise(res):
     kinds = [k for k in ("cold64", "warm64", "cold32", "warm32") if k in res[0]]
 

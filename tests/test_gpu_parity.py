@@ -154,7 +154,7 @@ def test_launch_config_reports_the_choices(monkeypatch):
     from dgbgk.solver import Solver
     case = cylinder_small()
     print("case launch configuration", flush=True)
-    for prec, depth in ((_lib.DGBGK_FP64, 1), (_lib.DGBGK_FP32, 2)):
+    for prec, depth in ((_lib.DGBGK_FP64, 1), (_lib.DGBGK_FP32, 1)):
         c = Solver(case, precision=prec).launch_config()
         assert c["slots"] == 256 and c["depth"] == depth
         assert c["ntiles"] >= -(-case.mesh.nelem // 256)
