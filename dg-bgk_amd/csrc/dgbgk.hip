@@ -1795,15 +1795,7 @@ static bool reorder_enabled() {
   return !(r && r[0] == '0');
 }
 
-static Params to_params(const dgbgk_params* prm) {
-  Params q;
-  q.rv = prm->rv; q.csafm = prm->csafm; q.alpha = prm->alpha; q.rgas = prm->rgas;
-  q.d = prm->d; q.m = prm->m; q.t1 = prm->t1; q.p1 = prm->p1; q.u0 = prm->u0; q.v0 = prm->v0;
-  for (int k = 0; k < 4; ++k) q.cinf[k] = prm->cinf[k];
-  q.inf = prm->inf; q.immat = prm->immat; q.precision = prm->precision;
-  q.lookup = prm->lookup; q.order = prm->order; q.vchunk = prm->vchunk;
-  return q;
-}
+static Params to_params(const dgbgk_params* prm) { return params_from_api(*prm); }
 
 extern "C" {
 
@@ -1824,20 +1816,18 @@ int dgbgk_setup_create(int nelem, int npoin, int nboun, const int* intma, const 
     delete s;
     return fail(st, msg);
   }
-  s->ntiles = (int)s->p.tile_e0.size();
+  // the tiles a device context builds, on its storage order
   std::vector<int> order;
-  if (reorder_enabled() && hilbert_order(nelem, npoin, intma, coord, order)) {
-    std::vector<int> im, bs;
-    renumber_elements(nelem, nboun, intma, bsido, order, im, bs);
-    Prep q;
-    st = prepare(nelem, npoin, nboun, im.data(), coord, bs.data(), rorder, torder, eta, w,
-                 to_params(prm), kSlots, kSlots, q, msg);
-    if (st) {
-      delete s;
-      return fail(st, msg);
-    }
-    s->ntiles = (int)q.tile_e0.size();
+  Prep q;
+  const char* bo = std::getenv("DGBGK_BORDER_ORDER");
+  st = prepare_storage(nelem, npoin, nboun, intma, coord, bsido, rorder, torder, eta, w,
+                       to_params(prm), kSlots, kSlots, reorder_enabled(),
+                       reorder_enabled() && !(bo && bo[0] == '0'), q, order, msg);
+  if (st) {
+    delete s;
+    return fail(st, msg);
   }
+  s->ntiles = (int)q.tile_e0.size();
   *out = s;
   return kOk;
 }
@@ -1873,6 +1863,21 @@ int dgbgk_setup_tables_host(dgbgk_setup* s, int* iside, double* nx, double* ny, 
   return kOk;
 }
 
+int dgbgk_setup_reflections(dgbgk_setup* s, int* side, int* i1, int* i2, double* zd1,
+                            double* zd2) {
+  if (!s) return fail(kErrArg, "null setup");
+  const Prep& p = s->p;
+  const size_t n = p.refl_i1.size();
+  for (size_t k = 0; side && k < p.wall_sides.size(); ++k) side[k] = p.wall_sides[k] + 1;
+  for (size_t k = 0; k < n; ++k) {
+    if (i1) i1[k] = p.refl_i1[k] + 1;   // -1 (no reflection) becomes 0
+    if (i2) i2[k] = p.refl_i2[k] + 1;
+    if (zd1) zd1[k] = p.refl_zd1[k];
+    if (zd2) zd2[k] = p.refl_zd2[k];
+  }
+  return kOk;
+}
+
 void dgbgk_setup_destroy(dgbgk_setup* s) { delete s; }
 
 int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double* coord,
@@ -1895,47 +1900,17 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   if (const char* ps = std::getenv("DGBGK_SLOTS")) c.slots = std::atoi(ps) == 512 ? 512 : 256;
   int owned_cap = c.slots;
   if (const char* po = std::getenv("DGBGK_TILE_OWNED")) owned_cap = std::atoi(po);
-  // the device works on the mesh renumbered into storage order (compact
-  // tiles); the ABI's per-element arrays stay in the input numbering
-  const int* im = intma;
-  const int* bs = bsido;
-  std::vector<int> im_r, bs_r;
-  if (reorder_enabled() && hilbert_order(nelem, npoin, intma, coord, c.eorder)) {
-    renumber_elements(nelem, nboun, intma, bsido, c.eorder, im_r, bs_r);
-    im = im_r.data();
-    bs = bs_r.data();
-  }
   Params dprm = to_params(prm);
   if (const char* ta = std::getenv("DGBGK_TILE_ALIGN")) dprm.tile_align = std::max(1, std::atoi(ta));
-  int st = prepare(nelem, npoin, nboun, im, coord, bs, rorder, torder, eta, w, dprm, c.slots,
-                   owned_cap, c.p, msg);
+  const char* bo = std::getenv("DGBGK_BORDER_ORDER");
+  int st = prepare_storage(nelem, npoin, nboun, intma, coord, bsido, rorder, torder, eta, w,
+                           dprm, c.slots, owned_cap, reorder_enabled(),
+                           reorder_enabled() && !(bo && bo[0] == '0'), c.p, c.eorder, msg);
   if (st) {
     delete h;
     return fail(st, msg);
   }
-  // inside each tile, the elements that other tiles read as halo last,
-  // grouped by reader; the same tiles are rebuilt on that order
-  const char* bo = std::getenv("DGBGK_BORDER_ORDER");
-  if (reorder_enabled() && !(bo && bo[0] == '0')) {
-    std::vector<int> perm, bounds;
-    border_order(c.p, perm, bounds);
-    bool moved = false;
-    for (int i = 0; i < nelem && !moved; ++i) moved = perm[i] != i;
-    if (moved) {
-      std::vector<int> fin(nelem);
-      for (int i = 0; i < nelem; ++i) fin[i] = c.eorder.empty() ? perm[i] : c.eorder[perm[i]];
-      c.eorder.swap(fin);
-      renumber_elements(nelem, nboun, intma, bsido, c.eorder, im_r, bs_r);
-      st = prepare(nelem, npoin, nboun, im_r.data(), coord, bs_r.data(), rorder, torder, eta, w,
-                   dprm, c.slots, owned_cap, c.p, msg, &bounds);
-      if (st) {
-        delete h;
-        return fail(st, msg);
-      }
-    }
-  }
   if (!c.eorder.empty()) {
-    csr_input_order(c.p, c.eorder);
     c.in.intma.assign(intma, intma + 3 * (size_t)nelem);
     c.in.bsido.assign(bsido, bsido + 6 * (size_t)std::max(nboun, 0));
     c.in.coord.assign(coord, coord + 2 * (size_t)npoin);

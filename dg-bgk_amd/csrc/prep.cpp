@@ -156,6 +156,16 @@ uint64_t pair_key(int a, int b) {
 
 }  // namespace
 
+Params params_from_api(const dgbgk_params& a) {
+  Params q;
+  q.rv = a.rv; q.csafm = a.csafm; q.alpha = a.alpha; q.rgas = a.rgas;
+  q.d = a.d; q.m = a.m; q.t1 = a.t1; q.p1 = a.p1; q.u0 = a.u0; q.v0 = a.v0;
+  for (int k = 0; k < 4; ++k) q.cinf[k] = a.cinf[k];
+  q.inf = a.inf; q.immat = a.immat; q.precision = a.precision;
+  q.lookup = a.lookup; q.order = a.order; q.vchunk = a.vchunk;
+  return q;
+}
+
 int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coord,
             const int* bsido, int rorder, int torder, const double* eta,
             const double* w, const Params& prm, int tile_slots, int tile_owned, Prep& p,
@@ -645,6 +655,46 @@ void csr_input_order(Prep& p, const std::vector<int>& order) {
   for (int q = 0; q < p.P; ++q)
     std::stable_sort(p.pt_node.begin() + p.pt_ptr[q], p.pt_node.begin() + p.pt_ptr[q + 1],
                      [&](int a, int b) { return order[a / 3] < order[b / 3]; });
+}
+
+int prepare_storage(int nelem, int npoin, int nboun, const int* intma, const double* coord,
+                    const int* bsido, int rorder, int torder, const double* eta,
+                    const double* w, const Params& prm, int tile_slots, int tile_owned,
+                    bool reorder, bool border, Prep& p, std::vector<int>& eorder,
+                    std::string& msg) {
+  // the device works on the mesh renumbered into storage order (compact
+  // tiles); the ABI's per-element arrays stay in the input numbering
+  eorder.clear();
+  const int* im = intma;
+  const int* bs = bsido;
+  std::vector<int> im_r, bs_r;
+  if (reorder && hilbert_order(nelem, npoin, intma, coord, eorder)) {
+    renumber_elements(nelem, nboun, intma, bsido, eorder, im_r, bs_r);
+    im = im_r.data();
+    bs = bs_r.data();
+  }
+  int st = prepare(nelem, npoin, nboun, im, coord, bs, rorder, torder, eta, w, prm, tile_slots,
+                   tile_owned, p, msg);
+  if (st) return st;
+  // inside each tile, the elements that other tiles read as halo last,
+  // grouped by reader; the same tiles are rebuilt on that order
+  if (border) {
+    std::vector<int> perm, bounds;
+    border_order(p, perm, bounds);
+    bool moved = false;
+    for (int i = 0; i < nelem && !moved; ++i) moved = perm[i] != i;
+    if (moved) {
+      std::vector<int> fin(nelem);
+      for (int i = 0; i < nelem; ++i) fin[i] = eorder.empty() ? perm[i] : eorder[perm[i]];
+      eorder.swap(fin);
+      renumber_elements(nelem, nboun, intma, bsido, eorder, im_r, bs_r);
+      st = prepare(nelem, npoin, nboun, im_r.data(), coord, bs_r.data(), rorder, torder, eta,
+                   w, prm, tile_slots, tile_owned, p, msg, &bounds);
+      if (st) return st;
+    }
+  }
+  if (!eorder.empty()) csr_input_order(p, eorder);
+  return kOk;
 }
 
 }  // namespace dgbgk

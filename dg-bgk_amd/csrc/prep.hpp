@@ -7,6 +7,8 @@
 #include <string>
 #include <vector>
 
+#include "../../include/dgbgk.h"
+
 namespace dgbgk {
 
 // Reference constants (e.g. src/advnce.f90:94, src/gteqnf.f90:18).
@@ -86,6 +88,9 @@ struct Prep {
   std::vector<int> nbr_slot;  // 3 per element: LDS slot of the neighbour
 };
 
+// The C-ABI parameter block (include/dgbgk.h) as the host setup's Params.
+Params params_from_api(const dgbgk_params& a);
+
 // Builds everything; returns a Status and fills msg on error.
 int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coord,
             const int* bsido, int rorder, int torder, const double* eta,
@@ -123,5 +128,15 @@ void border_order(const Prep& p, std::vector<int>& perm, std::vector<int>& bound
 // Point CSR of a renumbered mesh back in input element order, the order in
 // which MACROS accumulates the point averages (src/macros.f90:156-170).
 void csr_input_order(Prep& p, const std::vector<int>& order);
+
+// The whole host setup of a device context: the storage order (the Hilbert
+// order when reorder, refined by border_order when border), the tables
+// built on it, and the point CSR back in input order.  eorder: storage
+// position -> input element, 0-based, empty when the input order is kept.
+int prepare_storage(int nelem, int npoin, int nboun, const int* intma, const double* coord,
+                    const int* bsido, int rorder, int torder, const double* eta,
+                    const double* w, const Params& prm, int tile_slots, int tile_owned,
+                    bool reorder, bool border, Prep& p, std::vector<int>& eorder,
+                    std::string& msg);
 
 }  // namespace dgbgk

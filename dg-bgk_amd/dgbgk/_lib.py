@@ -31,6 +31,7 @@ EXPORTS = [
     "dgbgk_set_f", "dgbgk_seed_maxwellian", "dgbgk_node_fields", "dgbgk_point_fields",
     "dgbgk_forces", "dgbgk_wall_eta", "dgbgk_sizes", "dgbgk_setup_tables",
     "dgbgk_setup_create", "dgbgk_setup_sizes", "dgbgk_setup_tables_host",
+    "dgbgk_setup_reflections",
     "dgbgk_setup_destroy", "dgbgk_timing", "dgbgk_kernel_time", "dgbgk_sync",
     "dgbgk_dof_bytes", "dgbgk_launch_config", "dgbgk_f_stats", "dgbgk_f_stats_noalloc",
     "dgbgk_mlp_create",
@@ -107,6 +108,7 @@ def lib():
                                      C.c_int, _dp, _dp, C.POINTER(Params), C.POINTER(vp)]
     L.dgbgk_setup_sizes.argtypes = [vp] + [C.POINTER(C.c_int)] * 5
     L.dgbgk_setup_tables_host.argtypes = [vp, oip] + [odp] * 8 + [C.POINTER(C.c_double)] * 2
+    L.dgbgk_setup_reflections.argtypes = [vp, oip, oip, oip, odp, odp]
     L.dgbgk_setup_destroy.argtypes = [vp]
     L.dgbgk_setup_destroy.restype = None
     L.dgbgk_timing.argtypes = [vp, C.c_int]

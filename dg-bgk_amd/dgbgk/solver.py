@@ -61,6 +61,19 @@ class Setup:
                                             t["vcord"], C.byref(sw), C.byref(dc)))
         return _shape_tables(t, self.S, self.E, self.B, self.V, sw.value, dc.value)
 
+    def reflections(self) -> dict:
+        """GETBOU's reflection table: side (NW,) 1-based boundary sides; i1,
+        i2 (NW, V) 1-based bracketing ordinates, 0 where not reflected; zd1,
+        zd2 (NW, V) the ZETA bracket width and offset."""
+        NW, V = self.nwall, self.V
+        side = np.zeros(max(NW, 1), np.int32)
+        i1, i2 = np.zeros(max(NW * V, 1), np.int32), np.zeros(max(NW * V, 1), np.int32)
+        zd1, zd2 = np.zeros(max(NW * V, 1)), np.zeros(max(NW * V, 1))
+        check(lib().dgbgk_setup_reflections(self.h, side, i1, i2, zd1, zd2))
+        shp = (NW, V)
+        return dict(side=side[:NW], i1=i1[:NW * V].reshape(shp), i2=i2[:NW * V].reshape(shp),
+                    zd1=zd1[:NW * V].reshape(shp), zd2=zd2[:NW * V].reshape(shp))
+
 
 def _alloc_tables(S, E, B, V):
     return dict(iside=np.zeros(8 * S, np.int32), nx=np.zeros(S), ny=np.zeros(S),

@@ -165,6 +165,15 @@ int dgbgk_setup_tables_host(dgbgk_setup *s, int *iside, double *nx, double *ny,
                             double *el, double *geome, double *relen,
                             double *mmat, double *rsido, double *vcord,
                             double *sumweight, double *dtec);
+/* GETBOU's reflection table of the same setup (src/getbou.f90:190-250):
+ * side[k] is wall slot k's boundary side (1-based row of BSIDO), k <
+ * nwall_sides; for slot k and ordinate v (arrays nwall_sides x nvel,
+ * slot-major) i1/i2 are the 1-based ordinates the reflected velocity is
+ * interpolated between (0 where v leaves through the wall and is not
+ * reflected), zd1 the ZETA width of their bracket and zd2 the reflected
+ * ZETA's offset from i1's.  Any pointer may be NULL. */
+int dgbgk_setup_reflections(dgbgk_setup *s, int *side, int *i1, int *i2,
+                            double *zd1, double *zd2);
 void dgbgk_setup_destroy(dgbgk_setup *s);
 
 /* Kernel timing with HIP events on the context's stream.  Kernel ids:
