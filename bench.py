@@ -111,6 +111,36 @@ def time_to_residual():
             "drag_rel": abs(learned["drag"] - plain["drag"]) / abs(plain["drag"])}
 
 
+def network_bench(steps: int = 200):
+    """The warm start's network on its own: Adam steps at batch 65,536 and
+    inference of 2^20 points (5 -> 6 x 256 SiLU -> 4, BF16 MFMA kernels), on
+    seeded synthetic data.  Under rocprofv3 --kernel-trace this gives the
+    per-kernel times; one step's matrix products are 6 x 331,008 x 65,536
+    flop (forward, data and weight gradients)."""
+    import numpy as np
+    from dgbgk.mlp import MLP
+    rng = np.random.default_rng(0)
+    n = 1 << 20
+    X = rng.standard_normal((n, 5)).astype(np.float32)
+    Y = np.tanh(X[:, :4] * 0.7 + X[:, 1:5] * 0.3).astype(np.float32)
+    m = MLP(fin=5, width=256, hidden=6, fout=4, batch=65536)
+    m.init(0)
+    m.set_data(X, Y)
+    m.train(20, lr=1e-3, seed=1)
+    t0 = time.perf_counter()
+    loss = m.train(steps, lr=1e-3, seed=2)
+    dt = time.perf_counter() - t0
+    flop = 6.0 * (5 * 256 + 5 * 256 * 256 + 256 * 4) * 65536
+    t1 = time.perf_counter()
+    m.forward(X)
+    df = time.perf_counter() - t1
+    m.close()
+    return {"train_steps": steps, "batch": 65536, "ms_per_train_step": 1e3 * dt / steps,
+            "train_tflops": flop * steps / dt / 1e12, "loss_last": float(loss[-1]),
+            "forward_points": n, "forward_ms": 1e3 * df,
+            "forward_tflops": flop / 3 * (n / 65536) / df / 1e12}
+
+
 KERNELS = {0: "update", 1: "wall", 2: "moments", 3: "eta", 4: "final", 5: "points"}
 
 
@@ -276,7 +306,13 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run three device-to-device copies of BYTES bytes (the "
                          "reference kernel that scales rocprofv3's FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--network", type=int, default=0, metavar="STEPS",
+                    help="only time the warm start's network: STEPS training steps and one "
+                         "inference pass (for rocprofv3 kernel traces)")
     args = ap.parse_args()
+    if args.network:
+        print(json.dumps(network_bench(args.network)))
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

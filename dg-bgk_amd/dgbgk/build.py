@@ -76,9 +76,24 @@ def build_fortran_host(verbose: bool = False) -> str | None:
     return FHOST
 
 
+STREAM = os.path.join(PKG, "stream_ceiling")
+
+
+def build_stream_ceiling(verbose: bool = False) -> str:
+    """The streaming-bandwidth probe (csrc/stream_ceiling.hip), an executable."""
+    tmp = STREAM + ".tmp"
+    _run([HIPCC] + [f for f in HIP_FLAGS if f != "-fPIC"] +
+         [os.path.join(CSRC, "stream_ceiling.hip"), "-o", tmp])
+    os.replace(tmp, STREAM)
+    if verbose:
+        print("built", STREAM)
+    return STREAM
+
+
 def build_all(verbose: bool = False):
     build_lib(verbose)
     build_fortran_host(verbose)
+    build_stream_ceiling(verbose)
 
 
 if __name__ == "__main__":
