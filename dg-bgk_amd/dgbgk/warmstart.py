@@ -224,9 +224,10 @@ def _save(path, d):
 
 
 def generate(elems, n, seed, rorder, torder, tol, max_steps, data, time_budget=None,
-             log=print, geometry="cylinder"):
-    """Steady FP64 states of n drawn cases, appended to the npz at `data`
-    (resumable: cases already there are skipped)."""
+             log=print, geometry="cylinder", precision=_lib.DGBGK_FP64):
+    """Steady states of n drawn cases (FP64, or the FP32-bulk form, whose
+    steady fields differ from FP64's by ~1e-6), appended to the npz at
+    `data` (resumable: cases already there are skipped)."""
     mesh = mesh_for(elems, geometry)
     wd = meshgen.wall_distance(mesh)
     samples = draw(n, seed, geometry)
@@ -240,7 +241,8 @@ def generate(elems, n, seed, rorder, torder, tol, max_steps, data, time_budget=N
         if time_budget and time.perf_counter() - t0 > time_budget:
             log(f"  time budget reached after {i} cases")
             break
-        r = run_to_steady(cylinder_case(mesh, s, rorder, torder), tol, max_steps, log=log)
+        r = run_to_steady(cylinder_case(mesh, s, rorder, torder), tol, max_steps, precision,
+                          log=log)
         d["mach"] = np.append(d["mach"], s.mach)
         d["kn"] = np.append(d["kn"], s.kn)
         d["steps"] = np.append(d["steps"], r["steps"])
@@ -257,7 +259,10 @@ def generate(elems, n, seed, rorder, torder, tol, max_steps, data, time_budget=N
 
 
 def train(elems, data, params, steps, width=256, hidden=6, batch=65536, lr=1e-3, seed=0,
-          log=print, geometry="cylinder"):
+          log=print, geometry="cylinder", init=None):
+    """Adam on the steady states in `data`; from random weights, or from the
+    network at `init` (fine-tuning, e.g. on states of another velocity
+    grid)."""
     from .mlp import MLP
     mesh = mesh_for(elems, geometry)
     wd = meshgen.wall_distance(mesh)
@@ -268,7 +273,14 @@ def train(elems, data, params, steps, width=256, hidden=6, batch=65536, lr=1e-3,
     Y = np.concatenate([net_targets(y, s) for y, s in zip(d["y"][keep], samples)])
     log(f"  training set: {int(keep.sum())} steady states, {len(X)} points")
     m = MLP(fin=X.shape[1], width=width, hidden=hidden, fout=Y.shape[1], batch=batch)
-    m.init(seed=seed)
+    if init:
+        with np.load(init, allow_pickle=False) as z:
+            if (int(z["width"]), int(z["hidden"])) != (width, hidden):
+                raise ValueError(f"{init}: network shape differs from width {width}, "
+                                 f"hidden {hidden}")
+            m.set_params(z["params"])
+    else:
+        m.init(seed=seed)
     m.set_data(X, Y)
     losses = []
     t0 = time.perf_counter()
@@ -303,17 +315,19 @@ RUNS = {"warm64": (_lib.DGBGK_FP64, True), "cold32": (_lib.DGBGK_FP32, False),
 
 
 def evaluate(elems, params, n, seed, rorder, torder, tol, max_steps, log=print,
-             runs=("warm64", "cold32", "warm32"), ref_tol=None, geometry="cylinder"):
-    """Per held-out case: the FP64 run from the freestream (cold64) and the
-    requested others (FP64/FP32-bulk, from the freestream or from the
-    network's state), all judged by cold64's step-1 change.  With ref_tol,
-    cold64 goes on to change < ref_tol x its step-1 change, and every run's
-    drag is also compared with that converged drag (drag_rel_ref)."""
+             runs=("warm64", "cold32", "warm32"), ref_tol=None, geometry="cylinder",
+             samples=None):
+    """Per held-out case (n drawn with seed, or the given samples): the FP64
+    run from the freestream (cold64) and the requested others (FP64/FP32-
+    bulk, from the freestream or from the network's state), all judged by
+    cold64's step-1 change.  With ref_tol, cold64 goes on to change <
+    ref_tol x its step-1 change, and every run's drag is also compared with
+    that converged drag (drag_rel_ref)."""
     m = load_model(params)
     mesh = mesh_for(elems, geometry)
     wd = meshgen.wall_distance(mesh)
     out = []
-    for s in draw(n, seed, geometry):
+    for s in (samples if samples else draw(n, seed, geometry)):
         case = cylinder_case(mesh, s, rorder, torder)
         pred = untarget(m.forward(features(mesh, wd, s, geometry)), s)
         cold64 = run_to_steady(case, tol, max_steps, _lib.DGBGK_FP64, log=log, ref_tol=ref_tol)
@@ -440,6 +454,12 @@ def main(argv=None):
     ap.add_argument("--params", default="gpurun_out/warmstart_params.npz")
     ap.add_argument("--out", default="gpurun_out/warmstart_eval.json")
     ap.add_argument("--geometry", default="cylinder", choices=sorted(GEOMETRIES))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
+                    help="generate: the form the steady states are run in")
+    ap.add_argument("--init", default="",
+                    help="train: start from this network's weights (fine-tuning)")
+    ap.add_argument("--case", action="append", default=[], metavar="MACH,KN",
+                    help="evaluate: this case instead of drawn ones (repeatable)")
     ap.add_argument("--runs", default="warm64,cold32,warm32",
                     help="runs besides the FP64 freestream run, of warm64, cold32, warm32")
     a = ap.parse_args(argv)
@@ -447,19 +467,22 @@ def main(argv=None):
     os.makedirs(os.path.dirname(os.path.abspath(a.data)), exist_ok=True)
     if a.cmd == "generate":
         d = generate(a.elems, a.n, a.seed, a.rorder, a.torder, a.tol, a.max_steps, a.data,
-                     a.time_budget or None, log, a.geometry)
+                     a.time_budget or None, log, a.geometry,
+                     _lib.DGBGK_FP32 if a.precision == "fp32" else _lib.DGBGK_FP64)
         log(json.dumps(dict(cases=len(d["mach"]), converged=int(d["converged"].sum()),
                             median_steps=float(np.median(d["steps"])),
                             total_wall_s=float(d["wall_s"].sum()))))
     elif a.cmd == "train":
         _, losses, train_s = train(a.elems, a.data, a.params, a.train_steps, batch=a.batch,
-                                   lr=a.lr, log=log, geometry=a.geometry)
+                                   lr=a.lr, log=log, geometry=a.geometry,
+                                   init=a.init or None)
         log(json.dumps(dict(train_s=train_s, loss_first=float(losses[:20].mean()),
                             loss_last=float(losses[-50:].mean()))))
     else:
         res = evaluate(a.elems, a.params, a.n, a.seed, a.rorder, a.torder, a.tol,
                        a.max_steps, log, runs=tuple(x for x in a.runs.split(",") if x),
-                       ref_tol=a.ref_tol or None, geometry=a.geometry)
+                       ref_tol=a.ref_tol or None, geometry=a.geometry,
+                       samples=[Sample(*map(float, c.split(","))) for c in a.case])
         summ = summarise(res)
         with open(a.out, "w") as fh:
             json.dump(dict(elems=a.elems, ordinates=a.rorder * a.torder, tol=a.tol,
