@@ -21,16 +21,18 @@ on the library's stream.
 Extra keys (default run): "fp32_bulk", the same workload and measurement
 in the FP32-bulk form; "cylinder", the ~100k x 512 cylinder (Mach 0.5,
 Kn 0.05) in both forms.
-cpu_baseline: the FP64 C oracle, threaded over all host threads of the
-run (the same values bit for bit as its serial order), one step of a
+cpu_baseline: the FP64 C oracle, threaded over the run's OMP_NUM_THREADS
+(the same values bit for bit as its serial order), five steps of a
 bounded sample of the workload: the same mesh and flow with 2 x 64
 ordinates (DOF-velocity updates per second are per ordinate and element,
 so the rate carries over).
 time_to_residual (--no-ttr skips it): steps and wall time to a step change
-of 1e-6 of the plain run's step-1 change on the ~100k-triangle cylinder
-geometry held at the freestream on all box sides, FP64 from the freestream
-against FP32-bulk from the committed warm-start network's state (about a
-minute).
+of 1e-6 of the plain run's step-1 change, FP64 from the freestream against
+FP32-bulk from the committed warm-start network's state, freestream held
+on all box sides: the particle box at the workload's velocity grid and
+flow on a 25,781-triangle mesh ("time_to_residual"), and the ~100k-triangle
+cylinder ("time_to_residual_cylinder"); about a minute each.
+  python bench.py --network STEPS   times the warm start's network alone.
 """
 import argparse
 import json
@@ -74,41 +76,35 @@ def copy_calibration(nbytes: int):
     print(f"copied {nbytes} bytes x 3", file=sys.stderr)
 
 
-def time_to_residual():
-    """The metric's second half on the cylinder geometry: steps and wall time
-    to a step change of 1e-6 of the plain run's step-1 change, FP64 from the
-    freestream (plain) against FP32-bulk from the committed network's state
-    (learned), on the 101,753-triangle cylinder (16 x 32 ordinates, freestream
-    held on all box sides, as steady runs need).  One held-out case near the
-    median of the eight evaluated: Mach 0.82, Kn 0.17."""
-    from dgbgk import _lib, meshgen, warmstart
-    params = os.path.join(ROOT, "dg-bgk_amd", "dgbgk", "data", "warmstart_cylinder.npz")
-    mesh = warmstart.mesh_for(100_000)
-    smp = warmstart.Sample(0.82, 0.17)
-    case = warmstart.cylinder_case(mesh, smp, 16, 32)
-    log(f"# time to residual: cylinder E={mesh.nelem}, Mach {smp.mach}, Kn {smp.kn}")
-    plain = warmstart.run_to_steady(case, 1e-6, 400_000, _lib.DGBGK_FP64, log=log)
-    log(f"#   plain FP64: {plain['steps']} steps, {plain['wall_s']:.1f} s")
-    t0 = time.perf_counter()
-    m = warmstart.load_model(params)
-    pred = warmstart.untarget(m.forward(warmstart.features(mesh, meshgen.wall_distance(mesh), smp)),
-                              smp)
-    predict_s = time.perf_counter() - t0
-    learned = warmstart.run_to_steady(case, 1e-6, 400_000, _lib.DGBGK_FP32, pred,
-                                      plain["first_change"], log=log)
-    log(f"#   learned FP32-bulk: {learned['steps']} steps, {learned['wall_s']:.1f} s")
-    wl = learned["wall_s"] + predict_s
-    return {"case": f"cylinder {mesh.nelem} triangles x 16x32 ordinates, Mach 0.82, Kn 0.17, "
-                    "freestream on all box sides",
-            "tol": 1e-6,
-            "plain_fp64": {"steps": int(plain["steps"]), "wall_s": plain["wall_s"],
-                           "converged": bool(plain["converged"]), "drag": plain["drag"]},
-            "learned_fp32": {"steps": int(learned["steps"]), "wall_s": wl,
-                             "network_s": predict_s,
-                             "converged": bool(learned["converged"]), "drag": learned["drag"]},
-            "speedup_wall": plain["wall_s"] / wl,
-            "speedup_steps": plain["steps"] / max(learned["steps"], 1),
-            "drag_rel": abs(learned["drag"] - plain["drag"]) / abs(plain["drag"])}
+TTR_CASES = {
+    # geometry: (mesh triangles, rings, angles, Mach, Kn, network)
+    "particle": (23_000, 32, 64, 0.3, 1.0, "warmstart_particle.npz"),
+    "cylinder": (100_000, 16, 32, 0.82, 0.17, "warmstart_cylinder.npz"),
+}
+
+
+def time_to_residual(geometry: str):
+    """The metric's second half: steps and wall time to a step change of 1e-6
+    of the plain run's step-1 change, FP64 from the freestream (plain)
+    against FP32-bulk from the committed network's state (learned; its
+    inference counts), freestream held on all box sides as steady runs need.
+    particle: the workload's box, velocity grid (32 x 64) and flow (Mach 0.3,
+    Kn 1) on a 25,781-triangle mesh; the 1M-triangle mesh needs ~5x smaller
+    steps, ~1e5-1e6 of them at 30 ms, hours on one GPU.  The network was
+    fine-tuned on steady states of this grid; this case is held out.
+    cylinder: the 101,753-triangle cylinder, 16 x 32 ordinates, Mach 0.82,
+    Kn 0.17 (a held-out case near the median of eight evaluated)."""
+    from dgbgk import warmstart
+    elems, R, T, mach, kn, net = TTR_CASES[geometry]
+    mesh = warmstart.mesh_for(elems, geometry)
+    smp = warmstart.Sample(mach, kn)
+    case = warmstart.cylinder_case(mesh, smp, R, T)
+    log(f"# time to residual: {geometry} E={mesh.nelem} V={R * T}, Mach {mach}, Kn {kn}")
+    r = warmstart.time_to_residual(case, os.path.join(ROOT, "dg-bgk_amd", "dgbgk", "data", net),
+                                   smp, geometry, tol=1e-6, max_steps=400_000, log=log)
+    r["case"] = (f"{geometry} {mesh.nelem} triangles x {R}x{T} ordinates, Mach {mach}, Kn {kn}, "
+                 "freestream on all box sides")
+    return r
 
 
 def network_bench(steps: int = 200):
@@ -240,10 +236,11 @@ def host_threads() -> int:
 
 
 def cpu_baseline(config: str, elems: int, serial: bool) -> dict:
-    """The FP64 C oracle on a bounded sample of the workload: one step of the
-    same mesh and flow with 2 speed rings x 64 angles (Gauss-Legendre radial
-    nodes), threaded over the host threads (bit-identical to one thread);
-    optionally also on one thread."""
+    """The FP64 C oracle on a bounded sample of the workload: five steps of
+    the same mesh and flow with 2 speed rings x 64 angles (Gauss-Legendre
+    radial nodes; about 10 s on 16 threads at 1M triangles), threaded over
+    the host threads (bit-identical to one thread); optionally also one step
+    on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from dgbgk import cases
     from oracle_py import OracleRun
@@ -255,19 +252,22 @@ def cpu_baseline(config: str, elems: int, serial: bool) -> dict:
     dofv = 3 * case.mesh.nelem * case.nvel
     nt = host_threads()
     o.set_threads(nt)
+    nsteps = 5
     t0 = time.perf_counter()
-    o.step(1)
-    dt = time.perf_counter() - t0
+    o.step(nsteps)
+    dt = (time.perf_counter() - t0) / nsteps
     cpu = None
     for line in open("/proc/cpuinfo"):
         if line.startswith("model name"):
             cpu = line.split(":", 1)[1].strip()
             break
     out = {"value": dofv / dt, "unit": "DOF-velocity updates/s", "cores": nt, "kind": "port",
-           "sample": f"1 step of the {config} mesh (E={case.mesh.nelem}) and flow with 2 x 64 "
-                     f"ordinates ({dofv:.3e} DOF-velocity updates), FP64 C oracle in the "
-                     f"reference's order, {nt} threads, {dt:.1f} s",
-           "host": {"nproc": os.cpu_count(), "cpu": cpu or platform.processor()}}
+           "sample": f"{nsteps} steps of the {config} mesh (E={case.mesh.nelem}) and flow with "
+                     f"2 x 64 ordinates ({dofv:.3e} DOF-velocity updates per step), FP64 C "
+                     f"oracle in the reference's order, {nt} threads, {dt * nsteps:.1f} s",
+           "host": {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+                    "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+                    "cpu": cpu or platform.processor()}}
     if serial:
         o.set_threads(1)
         t0 = time.perf_counter()
@@ -373,7 +373,8 @@ def main():
     }
     out.update(extra)
     if not args.no_ttr and not args.only and plain and not args.elems:
-        out["time_to_residual"] = time_to_residual()
+        out["time_to_residual"] = time_to_residual("particle")
+        out["time_to_residual_cylinder"] = time_to_residual("cylinder")
     print(json.dumps(out))
 
 
