@@ -1456,17 +1456,13 @@ int update_blocks_per_cu_s(const Ctx& c) {
 
 template <typename T>
 int update_blocks_per_cu(const Ctx& c) {
-  return c.slots == 512   ? update_blocks_per_cu_s<T, 512>(c)
-         : c.slots == 384 ? update_blocks_per_cu_s<T, 384>(c)
-                          : update_blocks_per_cu_s<T, 256>(c);
+  return c.slots == 512 ? update_blocks_per_cu_s<T, 512>(c) : update_blocks_per_cu_s<T, 256>(c);
 }
 
 template <typename T>
 void launch_update(Ctx& c, const UpdArgs<T>& u) {
   if (c.slots == 512)
     launch_update_s<T, 512>(c, u);
-  else if (c.slots == 384)
-    launch_update_s<T, 384>(c, u);
   else
     launch_update_s<T, 256>(c, u);
 }
@@ -1901,10 +1897,7 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   // its own wavefront, which then skips phase B), on request; both
   // measured slower on gfx950.
   c.slots = kSlots;
-  if (const char* ps = std::getenv("DGBGK_SLOTS")) {
-    const int want = std::atoi(ps);
-    c.slots = want == 512 || want == 384 ? want : 256;
-  }
+  if (const char* ps = std::getenv("DGBGK_SLOTS")) c.slots = std::atoi(ps) == 512 ? 512 : 256;
   int owned_cap = c.slots;
   if (const char* po = std::getenv("DGBGK_TILE_OWNED")) owned_cap = std::atoi(po);
   Params dprm = to_params(prm);
