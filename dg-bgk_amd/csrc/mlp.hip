@@ -128,17 +128,64 @@ __global__ void __launch_bounds__(256) k_gemm(const bf16* __restrict__ A, int ld
   }
 }
 
-// sum over splits of the partial products -> FP32 gradient
-__global__ void k_reduce_partials(const float* __restrict__ P, int splits, size_t n,
-                                  float* __restrict__ G) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// sum over splits of partials: G[i] = sum_z P[z n + i].  A block takes 32
+// outputs x 8 split groups (coalesced 128-byte reads per group), the groups
+// combine in LDS.  With nin > 0 the outputs are k_wgrad_* partials,
+// [nout][nin + 1], scattered into the packed weight [nout][nin] and bias
+// [nout] gradients; with nin == 0 G is written directly.
+constexpr int kRedCols = 32, kRedGroups = 8;
+__global__ void __launch_bounds__(256) k_reduce_splits(const float* __restrict__ P, int splits,
+                                                       int n, int nin, float* __restrict__ G,
+                                                       float* __restrict__ Gb) {
+  __shared__ float s_sum[kRedGroups][kRedCols];
+  const int c = threadIdx.x % kRedCols, grp = threadIdx.x / kRedCols;
+  const int i = blockIdx.x * kRedCols + c;
   float s = 0.0f;
-  for (int z = 0; z < splits; ++z) s += P[(size_t)z * n + i];
-  G[i] = s;
+  if (i < n)
+    for (int z = grp; z < splits; z += kRedGroups) s += P[(size_t)z * n + i];
+  s_sum[grp][c] = s;
+  __syncthreads();
+  if (grp != 0 || i >= n) return;
+  float t = 0.0f;
+#pragma unroll
+  for (int g = 0; g < kRedGroups; ++g) t += s_sum[g][c];
+  if (nin == 0) {
+    G[i] = t;
+  } else {
+    const int o = i / (nin + 1), k = i % (nin + 1);
+    if (k < nin) G[o * nin + k] = t;
+    else Gb[o] = t;
+  }
 }
 
-// column sums of a bf16 [rows][cols] matrix (bias gradients), two stages
+// column sums of a bf16 [rows][cols] matrix over row chunks (bias
+// gradients): P[chunk][c].  Each thread reads 8 columns as one 16-byte
+// load; cols / 8 threads cover a row, the block's 256 / (cols / 8) row
+// lanes stride the chunk and combine in LDS (cols / 8 must divide 256).
+__global__ void __launch_bounds__(256) k_colsum8(const bf16* __restrict__ D, int rows, int cols,
+                                                 int rows_per, float* __restrict__ P) {
+  __shared__ float s_sum[256 * 8];
+  const int gpr = cols / 8, lanes = 256 / gpr;
+  const int cg = threadIdx.x % gpr, rl = threadIdx.x / gpr;
+  const int r0 = blockIdx.x * rows_per, r1 = min(rows, r0 + rows_per);
+  float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int r = r0 + rl; r < r1; r += lanes) {
+    const uint4 v = *reinterpret_cast<const uint4*>(D + (size_t)r * cols + 8 * cg);
+    const bf16* h = reinterpret_cast<const bf16*>(&v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[j] += bf2f(h[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s_sum[rl * cols + 8 * cg + j] = acc[j];
+  __syncthreads();
+  for (int c = threadIdx.x; c < cols; c += 256) {
+    float t = 0.0f;
+    for (int l = 0; l < lanes; ++l) t += s_sum[l * cols + c];
+    P[(size_t)blockIdx.x * cols + c] = t;
+  }
+}
+
+// the same for any cols: one thread per column
 __global__ void k_colsum_partial(const bf16* __restrict__ D, int rows, int cols, int rows_per,
                                  float* __restrict__ P) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -149,32 +196,78 @@ __global__ void k_colsum_partial(const bf16* __restrict__ D, int rows, int cols,
   P[(size_t)blockIdx.y * cols + c] = s;
 }
 
-// input layer (fin -> W): Z = X W^T + b, A = SiLU(Z); X fp32, weights bf16
-__global__ void k_in_fwd(const float* __restrict__ X, int fin, const bf16* __restrict__ W,
-                         const float* __restrict__ b, int width, int rows,
-                         bf16* __restrict__ Z, bf16* __restrict__ O) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)rows * width) return;
-  const int r = (int)(i / width), j = (int)(i % width);
-  float z = b[j];
-  for (int k = 0; k < fin; ++k) z += bf2f(W[j * fin + k]) * bf2f(f2bf(X[(size_t)r * fin + k]));
-  Z[i] = f2bf(z);
-  O[i] = f2bf(silu(z));
+// rows per block of the row-blocked layer kernels
+constexpr int kRowBlock = 16;
+// widest input and output layers the small-layer kernels keep in registers
+constexpr int kMaxIO = 8;
+
+// input layer (fin -> W): Z = X W^T + b, A = SiLU(Z); X fp32 (rounded to
+// bf16 as the GEMMs' operands are), weights bf16.  A block takes kRowBlock
+// rows; a thread keeps its column's fin weights in registers.
+__global__ void __launch_bounds__(256) k_in_fwd(const float* __restrict__ X, int fin,
+                                                const bf16* __restrict__ W,
+                                                const float* __restrict__ b, int width,
+                                                int rows, bf16* __restrict__ Z,
+                                                bf16* __restrict__ O) {
+  const int r0 = blockIdx.x * kRowBlock, r1 = min(rows, r0 + kRowBlock);
+  for (int j = threadIdx.x; j < width; j += blockDim.x) {
+    float w[kMaxIO];
+#pragma unroll
+    for (int k = 0; k < kMaxIO; ++k) w[k] = k < fin ? bf2f(W[j * fin + k]) : 0.0f;
+    const float bj = b[j];
+    for (int r = r0; r < r1; ++r) {
+      float z = bj;
+#pragma unroll
+      for (int k = 0; k < kMaxIO; ++k)
+        if (k < fin) z += w[k] * bf2f(f2bf(X[(size_t)r * fin + k]));
+      const size_t i = (size_t)r * width + j;
+      Z[i] = f2bf(z);
+      O[i] = f2bf(silu(z));
+    }
+  }
 }
 
-// output layer (W -> fout) and MSE loss; dY = 2 (Y - T) / (rows * fout)
-__global__ void k_out_fwd(const bf16* __restrict__ Ain, int width, const bf16* __restrict__ W,
-                          const float* __restrict__ b, int fout, int rows,
-                          float* __restrict__ Y, const float* __restrict__ Tgt,
-                          float* __restrict__ dY, float* __restrict__ loss_part) {
+// output layer (W -> fout) and MSE loss; dY = 2 (Y - T) / (rows * fout).
+// Four lanes per row, each a quarter of the row's width in 16-byte loads;
+// the quarters combine by lane exchange.  64 rows per block.
+__global__ void __launch_bounds__(256) k_out_fwd(const bf16* __restrict__ Ain, int width,
+                                                 const bf16* __restrict__ W,
+                                                 const float* __restrict__ b, int fout,
+                                                 int rows, float* __restrict__ Y,
+                                                 const float* __restrict__ Tgt,
+                                                 float* __restrict__ dY,
+                                                 float* __restrict__ loss_part) {
   __shared__ float s_red[4];
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  float l = 0.0f;
+  const int q = threadIdx.x & 3;
+  const int r = blockIdx.x * 64 + (threadIdx.x >> 2);
+  const int qw = width / 4, k0 = q * qw;
+  float acc[kMaxIO];
+#pragma unroll
+  for (int o = 0; o < kMaxIO; ++o) acc[o] = 0.0f;
   if (r < rows) {
+    const bf16* a = Ain + (size_t)r * width + k0;
+    for (int k = 0; k < qw; k += 8) {
+      const uint4 va = *reinterpret_cast<const uint4*>(a + k);
+      const bf16* ha = reinterpret_cast<const bf16*>(&va);
+#pragma unroll
+      for (int o = 0; o < kMaxIO; ++o) {
+        if (o >= fout) break;
+        const uint4 vw = *reinterpret_cast<const uint4*>(W + (size_t)o * width + k0 + k);
+        const bf16* hw = reinterpret_cast<const bf16*>(&vw);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[o] += bf2f(hw[j]) * bf2f(ha[j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < kMaxIO; ++o) {
+    acc[o] += __shfl_xor(acc[o], 1, 64);
+    acc[o] += __shfl_xor(acc[o], 2, 64);
+  }
+  float l = 0.0f;
+  if (r < rows && q == 0) {
     for (int o = 0; o < fout; ++o) {
-      float y = b[o];
-      const bf16* a = Ain + (size_t)r * width;
-      for (int k = 0; k < width; ++k) y += bf2f(W[o * width + k]) * bf2f(a[k]);
+      const float y = b[o] + acc[o];
       Y[(size_t)r * fout + o] = y;
       if (Tgt) {
         const float d = y - Tgt[(size_t)r * fout + o];
@@ -194,41 +287,82 @@ __global__ void k_out_fwd(const bf16* __restrict__ Ain, int width, const bf16* _
   }
 }
 
-// output-layer backward: dA = dY W (times SiLU'(Zin)) -> bf16 dZ of the last
-// hidden layer; weight/bias gradients of the output layer as per-block partials
-__global__ void k_out_bwd(const float* __restrict__ dY, int fout, const bf16* __restrict__ W,
-                          int width, int rows, const bf16* __restrict__ Zin,
-                          bf16* __restrict__ dZ) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)rows * width) return;
-  const int r = (int)(i / width), k = (int)(i % width);
-  float g = 0.0f;
-  for (int o = 0; o < fout; ++o) g += dY[(size_t)r * fout + o] * bf2f(W[o * width + k]);
-  dZ[i] = f2bf(g * dsilu(bf2f(Zin[i])));
+// output-layer backward: dZ = (dY W) * SiLU'(Zin), bf16, for the last
+// hidden layer.  kRowBlock rows per block, a column's fout weights in
+// registers.
+__global__ void __launch_bounds__(256) k_out_bwd(const float* __restrict__ dY, int fout,
+                                                 const bf16* __restrict__ W, int width,
+                                                 int rows, const bf16* __restrict__ Zin,
+                                                 bf16* __restrict__ dZ) {
+  const int r0 = blockIdx.x * kRowBlock, r1 = min(rows, r0 + kRowBlock);
+  for (int k = threadIdx.x; k < width; k += blockDim.x) {
+    float w[kMaxIO];
+#pragma unroll
+    for (int o = 0; o < kMaxIO; ++o) w[o] = o < fout ? bf2f(W[o * width + k]) : 0.0f;
+    for (int r = r0; r < r1; ++r) {
+      float g = 0.0f;
+#pragma unroll
+      for (int o = 0; o < kMaxIO; ++o)
+        if (o < fout) g += dY[(size_t)r * fout + o] * w[o];
+      const size_t i = (size_t)r * width + k;
+      dZ[i] = f2bf(g * dsilu(bf2f(Zin[i])));
+    }
+  }
 }
 
-// gradient of a small dense layer: G[o][k] = sum_r D[r][o] * X[r][k],
-// G_b[o] = sum_r D[r][o]; D fp32 or bf16, X fp32 or bf16 (one thread per
-// (o,k) pair and row chunk), partials over row chunks
-template <typename TD, typename TX>
-__global__ void k_small_wgrad(const TD* __restrict__ D, int nout, const TX* __restrict__ X,
-                              int nin, int rows, int rows_per, float* __restrict__ P) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // o*(nin+1)+k, k==nin: bias
-  if (i >= nout * (nin + 1)) return;
-  const int o = i / (nin + 1), k = i % (nin + 1);
-  const int r0 = blockIdx.y * rows_per, r1 = min(rows, r0 + rows_per);
-  float s = 0.0f;
-  for (int r = r0; r < r1; ++r) {
-    float d, x = 1.0f;
-    if constexpr (sizeof(TD) == 4) d = D[(size_t)r * nout + o];
-    else d = bf2f(D[(size_t)r * nout + o]);
-    if (k < nin) {
-      if constexpr (sizeof(TX) == 4) x = bf2f(f2bf(X[(size_t)r * nin + k]));
-      else x = bf2f(X[(size_t)r * nin + k]);
+// weight and bias gradient partials of the output layer over one row chunk
+// per block: P[chunk][o (width + 1) + k] = sum_r dY[r][o] A[r][k] (k <
+// width), sum_r dY[r][o] (k == width).  A thread per column k of A (16-bit
+// loads coalesced across the block), all fout outputs in registers.
+__global__ void __launch_bounds__(256) k_wgrad_out(const float* __restrict__ dY, int fout,
+                                                   const bf16* __restrict__ A, int width,
+                                                   int rows, int rows_per,
+                                                   float* __restrict__ P) {
+  const int r0 = blockIdx.x * rows_per, r1 = min(rows, r0 + rows_per);
+  float* Pc = P + (size_t)blockIdx.x * fout * (width + 1);
+  for (int k = threadIdx.x; k < width; k += blockDim.x) {
+    float acc[kMaxIO];
+#pragma unroll
+    for (int o = 0; o < kMaxIO; ++o) acc[o] = 0.0f;
+    for (int r = r0; r < r1; ++r) {
+      const float x = bf2f(A[(size_t)r * width + k]);
+#pragma unroll
+      for (int o = 0; o < kMaxIO; ++o)
+        if (o < fout) acc[o] += dY[(size_t)r * fout + o] * x;
     }
-    s += d * x;
+    for (int o = 0; o < fout; ++o) Pc[o * (width + 1) + k] = acc[o];
   }
-  P[(size_t)blockIdx.y * nout * (nin + 1) + i] = s;
+  for (int o = threadIdx.x; o < fout; o += blockDim.x) {
+    float s = 0.0f;
+    for (int r = r0; r < r1; ++r) s += dY[(size_t)r * fout + o];
+    Pc[o * (width + 1) + width] = s;
+  }
+}
+
+// weight and bias gradient partials of the input layer over one row chunk
+// per block: P[chunk][o (fin + 1) + k] = sum_r dZ[r][o] X[r][k] (k < fin,
+// X rounded to bf16 as in the forward pass), sum_r dZ[r][o] (k == fin).  A
+// thread per column o of dZ.
+__global__ void __launch_bounds__(256) k_wgrad_in(const bf16* __restrict__ dZ, int width,
+                                                  const float* __restrict__ X, int fin,
+                                                  int rows, int rows_per,
+                                                  float* __restrict__ P) {
+  const int r0 = blockIdx.x * rows_per, r1 = min(rows, r0 + rows_per);
+  float* Pc = P + (size_t)blockIdx.x * width * (fin + 1);
+  for (int o = threadIdx.x; o < width; o += blockDim.x) {
+    float acc[kMaxIO + 1];
+#pragma unroll
+    for (int k = 0; k <= kMaxIO; ++k) acc[k] = 0.0f;
+    for (int r = r0; r < r1; ++r) {
+      const float d = bf2f(dZ[(size_t)r * width + o]);
+#pragma unroll
+      for (int k = 0; k < kMaxIO; ++k)
+        if (k < fin) acc[k] += d * bf2f(f2bf(X[(size_t)r * fin + k]));
+      acc[kMaxIO] += d;
+    }
+    for (int k = 0; k < fin; ++k) Pc[o * (fin + 1) + k] = acc[k];
+    Pc[o * (fin + 1) + fin] = acc[kMaxIO];
+  }
 }
 
 // Adam on FP32 master parameters, then the BF16 copy the kernels read
@@ -265,20 +399,6 @@ __global__ void k_gather(const float* __restrict__ X, const float* __restrict__ 
   const long idx = (long)(h % (unsigned long long)ns);
   for (int k = 0; k < fin; ++k) Xb[(size_t)r * fin + k] = X[(size_t)idx * fin + k];
   for (int k = 0; k < fout; ++k) Yb[(size_t)r * fout + k] = Y[(size_t)idx * fout + k];
-}
-
-// reduce partials of k_small_wgrad ([splits][nout][nin+1]) into the packed
-// weight [nout][nin] and bias [nout] gradients
-__global__ void k_reduce_small(const float* __restrict__ P, int splits, int nout, int nin,
-                               float* __restrict__ Gw, float* __restrict__ Gb) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = nout * (nin + 1);
-  if (i >= n) return;
-  float s = 0.0f;
-  for (int z = 0; z < splits; ++z) s += P[(size_t)z * n + i];
-  const int o = i / (nin + 1), k = i % (nin + 1);
-  if (k < nin) Gw[o * nin + k] = s;
-  else Gb[o] = s;
 }
 
 __global__ void k_sum_loss(const float* __restrict__ part, int n, float* __restrict__ out) {
@@ -353,15 +473,15 @@ int wsplits(int rows) {
 int forward(dgbgk_mlp* h, int rows, bool with_loss) {
   const int W = h->width;
   const dim3 b256(256);
-  const size_t n = (size_t)rows * W;
-  hipLaunchKernelGGL(k_in_fwd, dim3((unsigned)((n + 255) / 256)), b256, 0, h->stream, h->Xb,
-                     h->fin, h->pb + h->woff[0], h->p + h->boff[0], W, rows, h->Z[0], h->A[0]);
+  hipLaunchKernelGGL(k_in_fwd, dim3((rows + kRowBlock - 1) / kRowBlock), b256, 0, h->stream,
+                     h->Xb, h->fin, h->pb + h->woff[0], h->p + h->boff[0], W, rows, h->Z[0],
+                     h->A[0]);
   for (int l = 1; l < h->hidden; ++l)
     hipLaunchKernelGGL((k_gemm<false, false, kFwd>), dim3(W / kTile, rows / kTile, 1), b256,
                        0, h->stream, h->A[l - 1], W, h->pb + h->woff[l], W, W,
                        h->p + h->boff[l], h->Z[l], h->A[l], nullptr, nullptr, W, rows);
   const int L = h->nlayers - 1;
-  hipLaunchKernelGGL(k_out_fwd, dim3((rows + 255) / 256), b256, 0, h->stream,
+  hipLaunchKernelGGL(k_out_fwd, dim3((rows + 63) / 64), b256, 0, h->stream,
                      h->A[h->hidden - 1], W, h->pb + h->woff[L], h->p + h->boff[L], h->fout,
                      rows, h->Yout, with_loss ? h->Yb : nullptr, h->dY,
                      with_loss ? h->loss_part : nullptr);
@@ -377,13 +497,12 @@ int backward(dgbgk_mlp* h, int rows) {
   // output layer: weights and bias
   {
     const int n = h->fout * (W + 1);
-    hipLaunchKernelGGL((k_small_wgrad<float, bf16>), dim3((n + 255) / 256, nch), b256, 0,
-                       h->stream, h->dY, h->fout, h->A[H - 1], W, rows, rchunk, h->part);
-    hipLaunchKernelGGL(k_reduce_small, dim3((n + 255) / 256), b256, 0, h->stream, h->part, nch,
-                       h->fout, W, h->g + h->woff[L], h->g + h->boff[L]);
-    const size_t nz = (size_t)rows * W;
-    hipLaunchKernelGGL(k_out_bwd, dim3((unsigned)((nz + 255) / 256)), b256, 0, h->stream, h->dY,
-                       h->fout, h->pb + h->woff[L], W, rows, h->Z[H - 1], h->dZa);
+    hipLaunchKernelGGL(k_wgrad_out, dim3(nch), b256, 0, h->stream, h->dY, h->fout, h->A[H - 1],
+                       W, rows, rchunk, h->part);
+    hipLaunchKernelGGL(k_reduce_splits, dim3((n + kRedCols - 1) / kRedCols), b256, 0, h->stream,
+                       h->part, nch, n, W, h->g + h->woff[L], h->g + h->boff[L]);
+    hipLaunchKernelGGL(k_out_bwd, dim3((rows + kRowBlock - 1) / kRowBlock), b256, 0, h->stream,
+                       h->dY, h->fout, h->pb + h->woff[L], W, rows, h->Z[H - 1], h->dZa);
   }
   bf16* dz = h->dZa;
   bf16* dzn = h->dZb;
@@ -393,14 +512,17 @@ int backward(dgbgk_mlp* h, int rows) {
     hipLaunchKernelGGL((k_gemm<true, true, kPartial>), dim3(W / kTile, W / kTile, sp), b256, 0,
                        h->stream, dz, W, h->A[l - 1], W, rows, nullptr, nullptr, nullptr,
                        nullptr, h->part, W, W);
-    const size_t nw = (size_t)W * W;
-    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)((nw + 255) / 256)), b256, 0, h->stream,
-                       h->part, sp, nw, h->g + h->woff[l]);
+    const int nw = W * W;
+    hipLaunchKernelGGL(k_reduce_splits, dim3((nw + kRedCols - 1) / kRedCols), b256, 0,
+                       h->stream, h->part, sp, nw, 0, h->g + h->woff[l], nullptr);
     // db_l = column sums of dZ_l
-    hipLaunchKernelGGL(k_colsum_partial, dim3((W + 255) / 256, nch), b256, 0, h->stream, dz, rows,
-                       W, rchunk, h->part);
-    hipLaunchKernelGGL(k_reduce_partials, dim3((W + 255) / 256), b256, 0, h->stream, h->part,
-                       nch, (size_t)W, h->g + h->boff[l]);
+    if (256 % (W / 8) == 0)
+      hipLaunchKernelGGL(k_colsum8, dim3(nch), b256, 0, h->stream, dz, rows, W, rchunk, h->part);
+    else
+      hipLaunchKernelGGL(k_colsum_partial, dim3((W + 255) / 256, nch), b256, 0, h->stream, dz,
+                         rows, W, rchunk, h->part);
+    hipLaunchKernelGGL(k_reduce_splits, dim3((W + kRedCols - 1) / kRedCols), b256, 0, h->stream,
+                       h->part, nch, W, 0, h->g + h->boff[l], nullptr);
     // dZ_{l-1} = (dZ_l W_l) * SiLU'(Z_{l-1})
     hipLaunchKernelGGL((k_gemm<false, true, kBwdData>), dim3(W / kTile, rows / kTile, 1), b256,
                        0, h->stream, dz, W, h->pb + h->woff[l], W, W, nullptr, nullptr, dzn,
@@ -410,10 +532,10 @@ int backward(dgbgk_mlp* h, int rows) {
   // input layer
   {
     const int n = W * (h->fin + 1);
-    hipLaunchKernelGGL((k_small_wgrad<bf16, float>), dim3((n + 255) / 256, nch), b256, 0,
-                       h->stream, dz, W, h->Xb, h->fin, rows, rchunk, h->part);
-    hipLaunchKernelGGL(k_reduce_small, dim3((n + 255) / 256), b256, 0, h->stream, h->part, nch,
-                       W, h->fin, h->g + h->woff[0], h->g + h->boff[0]);
+    hipLaunchKernelGGL(k_wgrad_in, dim3(nch), b256, 0, h->stream, dz, W, h->Xb, h->fin, rows,
+                       rchunk, h->part);
+    hipLaunchKernelGGL(k_reduce_splits, dim3((n + kRedCols - 1) / kRedCols), b256, 0, h->stream,
+                       h->part, nch, n, h->fin, h->g + h->woff[0], h->g + h->boff[0]);
   }
   MCK(hipGetLastError());
   return 0;
@@ -442,9 +564,11 @@ extern "C" {
 const char* dgbgk_mlp_last_error(void) { return g_mlp_err.c_str(); }
 
 int dgbgk_mlp_create(int fin, int width, int hidden, int fout, int batch, dgbgk_mlp** out) {
-  if (!out || fin < 1 || fout < 1 || hidden < 2 || width % kTile != 0 || width < kTile ||
+  if (!out || fin < 1 || fin > kMaxIO || fout < 1 || fout > kMaxIO || hidden < 2 ||
+      width % kTile != 0 || width < kTile ||
       batch < kTile || batch % kTile != 0)
-    return mfail(DGBGK_ERR_ARG, "bad MLP shape (width and batch multiples of 64, hidden >= 2)");
+    return mfail(DGBGK_ERR_ARG, "bad MLP shape (1..8 inputs and outputs, width and batch "
+                                "multiples of 64, hidden >= 2)");
   *out = nullptr;
   auto* h = new (std::nothrow) dgbgk_mlp;
   if (!h) return mfail(DGBGK_ERR_NOMEM, "out of host memory");
@@ -493,7 +617,7 @@ int dgbgk_mlp_create(int fin, int width, int hidden, int fout, int batch, dgbgk_
   h->part_n = std::max({(size_t)64 * width * width, (size_t)nch * width * (fin + 1),
                         (size_t)nch * fout * (width + 1), (size_t)nch * width});
   st |= dalloc(h, &h->part, h->part_n);
-  st |= dalloc(h, &h->loss_part, (size_t)(batch + 255) / 256);
+  st |= dalloc(h, &h->loss_part, (size_t)(batch + 63) / 64);
   st |= dalloc(h, &h->losses, 1);
   if (st) return bail(st);
   if (hipMemset(h->m, 0, h->nparams * sizeof(float)) != hipSuccess ||
@@ -568,7 +692,7 @@ int dgbgk_mlp_loss_grad(dgbgk_mlp* h, int rows, const float* x, const float* y, 
   if ((st = forward(h, rows, true))) return st;
   if ((st = backward(h, rows))) return st;
   hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, h->stream, h->loss_part,
-                     (rows + 255) / 256, h->losses);
+                     (rows + 63) / 64, h->losses);
   if (loss)
     MCK(hipMemcpyAsync(loss, h->losses, sizeof(float), hipMemcpyDeviceToHost, h->stream));
   if (grad)
@@ -610,7 +734,7 @@ int dgbgk_mlp_train(dgbgk_mlp* h, int steps, float lr, unsigned seed, float* los
     if (!st) st = backward(h, rows);
     if (!st && dl)
       hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, h->stream, h->loss_part,
-                         (rows + 255) / 256, dl + s);
+                         (rows + 63) / 64, dl + s);
     if (!st) st = adam(h, lr);
   }
   if (!st && dl) {

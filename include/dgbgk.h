@@ -197,7 +197,8 @@ int dgbgk_launch_config(dgbgk_ctx *ctx, int *slots, int *depth, int *vchunk, int
  * fin -> width (SiLU), (hidden-1) x width -> width (SiLU), width -> fout.
  * BF16 weights and activations, FP32 accumulation (MFMA) and FP32 master
  * weights, Adam.  Parameters are packed per layer as W [dout][din] then
- * b [dout].  width and batch must be multiples of 64. */
+ * b [dout].  width and batch must be multiples of 64, fin and fout at
+ * most 8. */
 typedef struct dgbgk_mlp dgbgk_mlp;
 int dgbgk_mlp_create(int fin, int width, int hidden, int fout, int batch, dgbgk_mlp **out);
 void dgbgk_mlp_destroy(dgbgk_mlp *m);
