@@ -172,6 +172,7 @@ def from_run_dir(directory: str, deck: str = "run.con", single: bool = False) ->
     prm = dict(rv=r(keys["RV"]), csafm=r(keys["CSAFM"]), alpha=r(keys["ALPHA"]),
                rgas=r(keys["R"]), d=r(keys["D"]), m=r(keys["M"]), t1=r(keys["T1"]),
                p1=r(keys["P1"]), u0=r(keys["U0"]), v0=r(keys["V0"]),
-               cinf=[r(x) for x in keys["CINF"]], inf=int(keys["INF"]))
+               cinf=[r(x) for x in keys["CINF"]], inf=int(keys["INF"]),
+               immat=int(keys.get("IMMAT", 1)))
     return Case(os.path.basename(os.path.normpath(directory)), m, eta, w,
                 int(keys["TORDER"]), prm, nsteps=int(keys["NTIME"]))

@@ -270,7 +270,8 @@ def write_run_dir(case, directory: str, ntime: int, forceout: int = 1,
     p = case.prm
     keys = {
         "TORDER": int(case.torder), "rv": p["rv"], "FORCEOUT": int(forceout),
-        "NTIME": int(ntime), "IMMAT": 1, "ALPHA": p["alpha"], "CSAFM": p["csafm"],
+        "NTIME": int(ntime), "IMMAT": int(p.get("immat", 1)), "ALPHA": p["alpha"],
+        "CSAFM": p["csafm"],
         "RS": 1 if restart_in else 0, "T1": p["t1"], "P1": p["p1"], "U0": p["u0"], "V0": p["v0"],
         "INF": int(p.get("inf", 1)), "CINF": list(p["cinf"]), "NVSPACEPART": 1,
         "R": p["rgas"], "d": p["d"], "M": p["m"],

@@ -16,8 +16,8 @@ FC=${FC:-amdflang}
 [ -d "$REF" ] || { echo "reference sources not found at $REF; skipping _ref"; exit 0; }
 [ -f "$MPIINC/mpif.h" ] || { echo "mpif.h not found under $MPIINC; skipping _ref"; exit 0; }
 command -v "$FC" >/dev/null || { echo "$FC not found; skipping _ref"; exit 0; }
-SRCS="rfillv rfillm rfilla getsid normal getgeo getlen getmat getnor vspace gteqnf
-      getflu getfla getelc getbou getinc adthem inflow alotim alotim2 getfor"
+SRCS="rfillv rfillm rfilla getsid normal getgeo getlen getmat conmat getnor vspace gteqnf
+      getflu getfla getelc getbou getinc cmminc adthem inflow alotim alotim2 getfor"
 mkdir -p "$OUT"
 for prec in 64 32; do
   flags="-O2 -fPIC -I$MPIINC"

@@ -204,6 +204,23 @@ void ora_getlen(int nelem, const double *geome, double *relen)
   }
 }
 
+/* CONMAT: the inverse consistent mass matrix as the reference forms it,
+ * 0.75/(RJ/24) on the diagonal and -0.25/(RJ/24) off it (src/conmat.f90:1-44);
+ * CMMAT(IN,JN,IE) at (IN-1) + 3 (JN-1) + 9 (IE-1). */
+void ora_conmat(int nelem, const double *geome, double *cmmat)
+{
+  const double c24 = 1. / 24.;
+  for (int ie = 1; ie <= nelem; ie++) {
+    double rj = GEOME(7, ie);
+    double rj24 = rj * c24;
+    double rj1 = (1 / rj24);
+    double rj11 = 0.75 * rj1, rj12 = -0.25 * rj1;
+    for (int in = 1; in <= 3; in++)
+      for (int jn = 1; jn <= 3; jn++)
+        cmmat[(in - 1) + 3 * (jn - 1) + 9 * (size_t)(ie - 1)] = in == jn ? rj11 : rj12;
+  }
+}
+
 /* GETMAT: inverse lumped mass, 1/(4*RJ/24) (src/getmat.f90:8-40). */
 void ora_getmat(int nelem, const double *geome, double *mmat)
 {
@@ -258,6 +275,7 @@ struct ora_ctx {
   int E, P, B, S, R, T, V;
   int *intma, *bsido, *iside;
   double *coord, *nx, *ny, *el, *geome, *relen, *mmat, *rsido, *vcord;
+  double *cmmat; /* CONMAT's matrices (immat = 0 only) */
   double sumweight, dtec;
   ora_params prm;
   double cinf[4];
@@ -1084,9 +1102,34 @@ edges_done:
   /* RHS *= DTE (src/advnce.f90:342-346) */
 #pragma omp parallel for schedule(static) if (nt > 1) num_threads(nt)
   for (size_t k = 0; k < 3 * (size_t)E; k++) c->rhs[k] = AI(dte * c->rhs[k]);
+  double res = 0.0;
+  if (c->prm.immat == 0) {
+    /* CMMINC (src/cmminc.f90:20-41), as written: each node's increment adds
+     * CMMAT(IN,JN)*RHS(IN) over JN -- RHS(IN), not RHS(JN), so the sum is the
+     * matrix row sum (6/RJ, the lumped value) times RHS(IN) -- after the
+     * collision term; the residual is the largest signed CMMAT(IN,JN)*RHS(IN)
+     * + COLL*DTE, from 0 (no ABS) */
+#pragma omp parallel for schedule(static) reduction(max : res) if (nt > 1) num_threads(nt)
+    for (int ie = 1; ie <= E; ie++)
+      for (int in = 1; in <= 3; in++) {
+        size_t k = NE(in, ie);
+        double nf = F(c, in, iv, ie);
+        double coll = AI(P(c->mu[k]) * AI(c->nfo[k] - nf));
+        double d = AI(c->delun[k] + AI(coll * dte));
+        for (int jn = 1; jn <= 3; jn++) {
+          double cm = c->cmmat[(in - 1) + 3 * (jn - 1) + 9 * (size_t)(ie - 1)];
+          double rh = c->prm.mutant == ORA_MUTANT_CMMINC_RHS_JN ? c->rhs[NE(jn, ie)] : c->rhs[k];
+          double r1 = AI(AI(AI(cm) * rh) + AI(coll * dte));
+          if (r1 > res) res = r1;
+          d = AI(d + AI(AI(cm) * rh));
+        }
+        c->delun[k] = d;
+      }
+    c->residual[iv - 1] = res;
+    goto increments_done;
+  }
   /* GETINC (src/getinc.f90:22-43); the largest |increment| (a max, exact in
    * any order) */
-  double res = 0.0;
 #pragma omp parallel for schedule(static) reduction(max : res) if (nt > 1) num_threads(nt)
   for (int ie = 1; ie <= E; ie++)
     for (int in = 1; in <= 3; in++) {
@@ -1099,6 +1142,7 @@ edges_done:
       c->delun[k] = AI(AI(c->delun[k] + AI(AI(c->mmat[k]) * c->rhs[k])) + AI(coll * dte));
     }
   c->residual[iv - 1] = res;
+increments_done:
   if (g_stage) memcpy(g_stage->delun, c->delun, 3 * (size_t)E * sizeof(double));
   /* ADTHEM (src/adthem.f90:17-26) */
 #pragma omp parallel for schedule(static) if (nt > 1) num_threads(nt)
@@ -1253,7 +1297,13 @@ int ora_create(int nelem, int npoin, int nboun, const int *intma_in,
   ora_normal(ns, c->iside, c->coord, c->nx, c->ny, c->el);
   ora_getgeo(nelem, c->intma, c->coord, c->geome);
   ora_getlen(nelem, c->geome, c->relen);
-  ora_getmat(nelem, c->geome, c->mmat);
+  /* dg_lin_convec: GETMAT for IMMAT = 1, else CONMAT (src/dg_lin_convec.f90:113-117) */
+  if (prm->immat == 0) {
+    c->cmmat = zal(9 * E, sizeof(double));
+    ora_conmat(nelem, c->geome, c->cmmat);
+  } else {
+    ora_getmat(nelem, c->geome, c->mmat);
+  }
   ora_getnor(nboun, c->bsido, c->coord, c->rsido);
   ora_vspace(rorder, torder, eta, w, c->vcord, &c->sumweight);
   /* ALOTIM (src/alotim.f90:12-18) with MXSPD = rv */
@@ -1342,7 +1392,7 @@ void ora_destroy(ora_ctx *c)
 {
   if (!c) return;
   void *p[] = {c->intma, c->bsido, c->iside, c->coord, c->nx, c->ny, c->el,
-               c->geome, c->relen, c->mmat, c->rsido, c->vcord, c->cxa, c->cya,
+               c->geome, c->relen, c->mmat, c->cmmat, c->rsido, c->vcord, c->cxa, c->cya,
                c->cxb, c->cyb, c->jac, c->f, c->fsnap, c->disnd, c->disux,
                c->disuy, c->disps, c->mu, c->eta_b, c->nd_pp, c->rho_pp,
                c->uvel_pp, c->vvel_pp, c->ps_pp, c->temp_pp, c->flag, c->nd,
@@ -1352,6 +1402,12 @@ void ora_destroy(ora_ctx *c)
                c->node_si, c->node_su, c->node_sv, c->node_sp};
   for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); i++) free(p[i]);
   free(c);
+}
+
+void ora_cmmat(const ora_ctx *c, double *cmmat)
+{
+  if (c->cmmat) memcpy(cmmat, c->cmmat, 9 * (size_t)c->E * sizeof(double));
+  else memset(cmmat, 0, 9 * (size_t)c->E * sizeof(double));
 }
 
 void ora_get_f(const ora_ctx *c, double *f)

@@ -27,6 +27,7 @@ void ora_normal(int nside, const int *iside, const double *coord,
 void ora_getgeo(int nelem, const int *intma, const double *coord, double *geome);
 void ora_getlen(int nelem, const double *geome, double *relen);
 void ora_getmat(int nelem, const double *geome, double *mmat);
+void ora_conmat(int nelem, const double *geome, double *cmmat); /* CMMAT(3,3,NELEM) */
 void ora_getnor(int nboun, const int *bsido, const double *coord, double *rsido);
 void ora_vspace(int rorder, int torder, const double *eta, const double *w,
                 double *vcord, double *sumweight);
@@ -43,6 +44,8 @@ typedef struct ora_params {
   int mutant;             /* 0 = the restatement; otherwise a planted mistake,
                              used only by the tests that show a pin catches it:
                              ORA_MUTANT_* below */
+  int immat;              /* IVD%IMMAT: 1 lumped mass (GETMAT, GETINC), 0 the
+                             consistent-mass branch (CONMAT, CMMINC) */
 } ora_params;
 
 /* Planted mistakes (tests only).  Each is a plausible misreading of the
@@ -51,6 +54,7 @@ typedef struct ora_params {
 #define ORA_MUTANT_SWAP_NODES 2   /* EDGFLXOPT reads ISIDE(5:6) swapped */
 #define ORA_MUTANT_GETMAC_EVERY 3 /* GETMAC's point gather every step, not every 1000th */
 #define ORA_MUTANT_RES_AFTER 4    /* residual taken after INFLOW's reset */
+#define ORA_MUTANT_CMMINC_RHS_JN 5 /* CMMINC multiplies CMMAT(IN,JN) by RHS(JN) */
 
 typedef struct ora_ctx ora_ctx;
 
@@ -71,6 +75,8 @@ double ora_sumweight(const ora_ctx *c);
 
 /* f in the reference layout (3,V,E). */
 void ora_get_f(const ora_ctx *c, double *f);
+/* CMMAT(3,3,NELEM) of a context created with immat = 0 (zeros otherwise). */
+void ora_cmmat(const ora_ctx *c, double *cmmat);
 void ora_set_f(ora_ctx *c, const double *f); /* re-runs the initial MACROS */
 
 /* Per-step record. */

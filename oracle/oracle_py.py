@@ -21,7 +21,7 @@ class OraParams(C.Structure):
                 ("t1", C.c_double), ("p1", C.c_double), ("u0", C.c_double),
                 ("v0", C.c_double), ("cinf", C.c_double * 4), ("inf", C.c_int),
                 ("lookup", C.c_int), ("jacobi", C.c_int), ("faithful_scans", C.c_int),
-                ("mutant", C.c_int)]
+                ("mutant", C.c_int), ("immat", C.c_int)]
 
 
 class OraStepInfo(C.Structure):
@@ -53,6 +53,10 @@ def lib():
         L.ora_sumweight.argtypes = [C.c_void_p]
         L.ora_sumweight.restype = C.c_double
         L.ora_get_f.argtypes = [C.c_void_p, _dp]
+        L.ora_cmmat.argtypes = [C.c_void_p, _dp]
+        L.ora_cmmat.restype = None
+        L.ora_conmat.argtypes = [C.c_int, _dp, _dp]
+        L.ora_conmat.restype = None
         L.ora_set_f.argtypes = [C.c_void_p, _dp]
         L.ora_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(OraStepInfo)]
         L.ora_step.restype = C.c_int
@@ -89,6 +93,7 @@ def lib():
 
 
 MUTANT_DOWNWIND, MUTANT_SWAP_NODES, MUTANT_GETMAC_EVERY, MUTANT_RES_AFTER = 1, 2, 3, 4
+MUTANT_CMMINC_RHS_JN = 5
 EMU_STORE_RN, EMU_STORE_SR, EMU_PARAMS, EMU_ARITH, EMU_DTE = 1, 2, 4, 8, 16
 EMU_PRED, EMU_EDGE, EMU_INC, EMU_FORM = 32, 64, 128, 256
 
@@ -103,6 +108,7 @@ def make_params(prm: dict, lookup: int = 1, jacobi: int = 0, faithful: int = 0,
     p.inf = int(prm.get("inf", 1))
     p.lookup, p.jacobi, p.faithful_scans = int(lookup), int(jacobi), int(faithful)
     p.mutant = int(mutant)
+    p.immat = int(prm.get("immat", 1))
     return p
 
 
@@ -153,6 +159,13 @@ class OracleRun:
     def set_emulation(self, flags: int):
         """FP32 emulation of stages of the step (EMU_* flags; 0 = FP64)."""
         lib().ora_set_emulation(self.h, int(flags))
+
+    def cmmat(self) -> np.ndarray:
+        """CONMAT's matrices, shape (E, 3, 3) as [IE][JN][IN] (zeros unless
+        the context was created with immat = 0)."""
+        m = np.empty(9 * self.E)
+        lib().ora_cmmat(self.h, m)
+        return m.reshape(self.E, 3, 3)
 
     def get_f(self) -> np.ndarray:
         """f in the reference layout, shape (E, V, 3)."""

@@ -10,7 +10,9 @@ src/getbou.f90:223-243, is defined):
                       outflow right, diffuse/specular walls (alpha 0.9)
                       top and bottom;
   cylinder_lobatto10  cylinder in a 20 x 10 box (2021 triangles), wall
-                      alpha 0.9.
+                      alpha 0.9;
+  lobatto10_walls_immat0  the first with IMMAT = 0, the consistent-mass
+                      branch (CONMAT, CMMINC).
 
 Each runs 1010 steps with FORCEOUT = 1, so RESIDUAL.RES holds every step's
 LOG(SUMRES/SRITM1) with the lift and drag of the last gathered pressure
@@ -46,8 +48,15 @@ import small_cases  # noqa: E402
 
 BGK = os.path.join(ROOT, "oracle", "_ref", "bgk32", "BGK")
 MPIEXEC = os.environ.get("MPIEXEC", "/opt/conda/bin/mpiexec")
+def _lobatto10_walls_immat0():
+    case = small_cases.lobatto10_walls()
+    case.prm["immat"] = 0
+    return case
+
+
 DECKS = {"lobatto10_walls": small_cases.lobatto10_walls,
-         "cylinder_lobatto10": small_cases.cylinder_lobatto10}
+         "cylinder_lobatto10": small_cases.cylinder_lobatto10,
+         "lobatto10_walls_immat0": _lobatto10_walls_immat0}
 NTIME = 1010
 KEEP = ("run.con", "mesh.plt", "velocity.txt", "RESIDUAL.RES", "RESULTS1.RES",
         "RESULTS2.RES", "FILE.OUT")
@@ -95,8 +104,10 @@ def run(name: str, make) -> dict:
 def main():
     if not os.path.exists(BGK):
         raise SystemExit(f"{BGK} not built: run bash oracle/build_ref_program.sh")
+    only = sys.argv[1:]
     for name, make in DECKS.items():
-        print(json.dumps(run(name, make)))
+        if not only or name in only:
+            print(json.dumps(run(name, make)))
 
 
 if __name__ == "__main__":

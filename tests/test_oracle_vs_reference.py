@@ -1,10 +1,10 @@
 """The C oracle against the reference's own routines (oracle/_ref, compiled
 from the reference's sources with -fdefault-real-8), bit for bit.
 
-Covers GETSID, NORMAL, GETGEO, GETLEN, GETMAT, GETNOR, VSPACE, ALOTIM,
-ALOTIM2, GTEQNF, GETFLA+GETELC, GETBOU (all branches, both ring rules and
-the in-place reads of already-advanced ordinates), GETINC+ADTHEM, INFLOW
-and GETFOR.  The routines that need MPI (FLUCON2, COMPMU, MACROS, the
+Covers GETSID, NORMAL, GETGEO, GETLEN, GETMAT, CONMAT, GETNOR, VSPACE,
+ALOTIM, ALOTIM2, GTEQNF, GETFLA+GETELC, GETBOU (all branches, both ring
+rules and the in-place reads of already-advanced ordinates), GETINC+ADTHEM,
+CMMINC+ADTHEM, INFLOW and GETFOR.  The routines that need MPI (FLUCON2, COMPMU, MACROS, the
 interior-side loop of EDGFLXOPT) are pinned in test_oracle_closed_forms.py.
 """
 import ctypes as C
@@ -192,16 +192,44 @@ def _interior_sides(rhs, f, umean, iside, nx, ny, el, iv, ux, uy):
         rhs[iel - 1, inl2 - 1] += rh[1]
 
 
-@pytest.mark.parametrize("make,lookup", [(lobatto10_walls, 0), (cylinder_lobatto10, 0)])
-def test_one_ordinate_stage_by_stage(make, lookup):
+@pytest.mark.parametrize("make", [c1, cylinder_small])
+def test_conmat_bit_exact(make):
+    """CONMAT (src/conmat.f90:1-44) against the reference's own, on the
+    reference's GEOME; the oracle context built with IMMAT = 0 holds the same
+    matrices."""
+    from oracle_py import OracleRun, lib
+    case = make()
+    case.prm["immat"] = 0
+    R = _ref()
+    E = case.mesh.nelem
+    t = _ref_setup(R, case.mesh)
+    geome = np.ascontiguousarray(t["geome"])
+    ref = np.zeros(9 * E)
+    R.L.ref_conmat(C.c_int(E), _ptr(geome), _ptr(ref))
+    ora = np.zeros(9 * E)
+    lib().ora_conmat(E, geome.ravel(), ora)
+    assert np.array_equal(ora, ref)
+    o = OracleRun(case)
+    assert np.array_equal(o.cmmat().ravel(), ref)
+    o.close()
+    # the row sums are the lumped inverse mass 6/RJ (GETMAT's value)
+    rows = ref.reshape(E, 3, 3).sum(axis=1)
+    assert np.allclose(rows, t["mmat"], rtol=1e-13)
+
+
+@pytest.mark.parametrize("make,lookup,immat", [(lobatto10_walls, 0, 1), (cylinder_lobatto10, 0, 1),
+                                               (lobatto10_walls, 0, 0), (cylinder_lobatto10, 0, 0)])
+def test_one_ordinate_stage_by_stage(make, lookup, immat):
     """Every ordinate of one step, in order, composed from the reference's
-    GTEQNF, GETFLA+GETELC, GETBOU, GETINC+ADTHEM and INFLOW on the same
-    arrays (so reflections read ordinates already advanced this step),
-    against the oracle.  Grids where the reference's ring rule is defined
-    (RORDER == TORDER, first node -1)."""
+    GTEQNF, GETFLA+GETELC, GETBOU, GETINC+ADTHEM (or, with IMMAT = 0,
+    CMMINC+ADTHEM on CONMAT's matrices) and INFLOW on the same arrays (so
+    reflections read ordinates already advanced this step), against the
+    oracle.  Grids where the reference's ring rule is defined (RORDER ==
+    TORDER, first node -1)."""
     from oracle_py import OracleRun
     case = make()
     case.prm["alpha"] = 0.6  # both the diffuse and the specular part of GETBOU
+    case.prm["immat"] = immat
     R = _ref()
     o = OracleRun(case, lookup=lookup)
     f = _perturbed(o).copy()
@@ -216,6 +244,8 @@ def test_one_ordinate_stage_by_stage(make, lookup):
     rsido = np.ascontiguousarray(t["rsido"])
     geome = np.ascontiguousarray(t["geome"])
     mmat = np.ascontiguousarray(t["mmat"])
+    cmmat = np.zeros(9 * case.mesh.nelem)
+    R.L.ref_conmat(C.c_int(case.mesh.nelem), _ptr(geome), _ptr(cmmat))
     mu = np.ascontiguousarray(nf["mu"])
     cinf = o.cinf()
     walls = 0
@@ -251,8 +281,14 @@ def test_one_ordinate_stage_by_stage(make, lookup):
         assert np.array_equal(rhs.ravel(), st["rhs_edge"]), iv
         rhs = dte * rhs
         residual = np.zeros(V)
-        R.L.ref_getinc_adthem(C.c_int(E), C.c_int(V), C.c_int(iv), _ptr(mmat), _ptr(rhs),
-                              _ptr(f), _ptr(nfo), _ptr(mu), C.c_double(dte), _ptr(residual))
+        if immat:
+            R.L.ref_getinc_adthem(C.c_int(E), C.c_int(V), C.c_int(iv), _ptr(mmat), _ptr(rhs),
+                                  _ptr(f), _ptr(nfo), _ptr(mu), C.c_double(dte),
+                                  _ptr(residual))
+        else:
+            R.L.ref_cmminc_adthem(C.c_int(E), C.c_int(V), C.c_int(iv), _ptr(cmmat), _ptr(rhs),
+                                  _ptr(f), _ptr(nfo), _ptr(mu), C.c_double(dte),
+                                  _ptr(residual))
         assert residual[iv - 1] == st["residual"], iv
         R.L.ref_inflow(C.c_int(iv), C.c_double(ux), C.c_double(uy), C.c_int(E),
                        _ptr(_ip(case.mesh.intma)), _ptr(cinf), C.c_int(case.mesh.nboun),
