@@ -75,6 +75,25 @@ __device__ __forceinline__ T fload_nt(const T* p) {
   return __builtin_nontemporal_load(p);
 }
 
+// The increment of a node and its residual term, mdt = M DTE.  IMMAT = 1,
+// GETINC (src/getinc.f90:34-41): d = M RHS DTE + COLL DTE, residual |d|.
+// IMMAT = 0, CMMINC as the reference writes it (src/cmminc.f90:30-39): each
+// node adds C(k,j) RHS(k) DTE over j, C the element's CONMAT matrix, whose
+// row sum is M = 6/RJ (diagonal 3M, off-diagonal -M), so d has the same
+// form; the residual is the largest signed term C(k,j) RHS DTE + COLL DTE,
+// i.e. 3 M RHS DTE + COLL DTE or -M RHS DTE + COLL DTE, from 0 (no
+// absolute value).  The reference sums the three products one by one; here
+// the row sum is taken first (a rounding-level difference).
+template <typename T, typename R>
+__device__ __forceinline__ T node_increment(bool cmminc, T mdt, T rhs, T cold, R& res) {
+  const T d = mdt * rhs + cold;
+  if (!cmminc)
+    res = fmax(res, (R)fabs(d));
+  else
+    res = fmax(res, (R)fmax((T(3) * mdt) * rhs + cold, cold - mdt * rhs));
+  return d;
+}
+
 __device__ __forceinline__ float dexp(float x) { return __expf(x); }
 __device__ __forceinline__ double dexp(double x) { return exp(x); }
 
@@ -213,7 +232,7 @@ __device__ __forceinline__ void store4<double>(double* p, double a, double b, do
 // dither's scale (zero without it).
 // Contains the block's one barrier per ordinate, so every thread of the
 // block calls it for every v.
-template <typename T, bool kMom>
+template <typename T, bool kMom, bool kCM>
 __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotState<T>& s,
                                                 int v, T ux, T uy, T fin, float ph,
                                                 float dsc0, float dsc1,
@@ -334,7 +353,7 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
   if constexpr (sizeof(T) == 4) t = fmaf(__builtin_amdgcn_fractf(s.dph + ph), dsc0, dsc1);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    d[k] = s.mdt * rhs[k] + cold[k];
+    d[k] = node_increment(kCM, s.mdt, rhs[k], cold[k], res);
     T dk = d[k];
     if constexpr (sizeof(T) == 4)
       dk = fmaf(t, __uint_as_float(__float_as_uint(f[k]) & 0x7f800000u), dk);
@@ -355,7 +374,6 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
 #pragma unroll
     for (int k = 0; k < 3; ++k) cg[k] = fabs(fn[k] - f[k]);
   }
-  res = fmax(res, fmax(fmax(fabs(d[0]), fabs(d[1])), fabs(d[2])));
   chg = fmax(chg, fmax(fmax(cg[0], cg[1]), cg[2]));
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -386,7 +404,9 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
 template <typename T>
 constexpr int kMinWaves = sizeof(T) == 4 ? DGBGK_FP32_MIN_WAVES : 1;
 
-template <typename T, int kDepth, bool kMom, int kS>
+// kCM: IMMAT = 0 (the consistent-mass branch): CMMINC's increment and
+// residual (node_increment), launched at prefetch depth 1.
+template <typename T, int kDepth, bool kMom, int kS, bool kCM>
 __global__ void __launch_bounds__(kS) __attribute__((amdgpu_waves_per_eu(kMinWaves<T>, 8)))
 k_update(UpdArgs<T> a) {
   constexpr int kQ = kDepth + 1;
@@ -514,7 +534,7 @@ k_update(UpdArgs<T> a) {
             qcy[jn] = ldc(a.vtd, 6 * V + vn);
           }
         }
-        update_ordinate<T, kMom>(a, s, v, qx[j], qy[j], qi[j], qp[j], dsc0, dsc1, q[j], s_f[j], dte,
+        update_ordinate<T, kMom, kCM>(a, s, v, qx[j], qy[j], qi[j], qp[j], dsc0, dsc1, q[j], s_f[j], dte,
                                  res, chg, qw[j], qcx[j], qcy[j], m0, m1, m2);
       }
     }
@@ -658,8 +678,7 @@ __global__ void __launch_bounds__(256) k_wall(WallArgs<T> a) {
                             a.dither ? -0x1p-24f : 0.f);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const T d = mdt * rhs[k] + cold[k];
-        res = fmax(res, (double)fabs(d));
+        const T d = node_increment(a.cmminc != 0, mdt, rhs[k], cold[k], res);
         T dk = d;
         if constexpr (sizeof(T) == 4) {
           dk = fmaf(wt, __uint_as_float(__float_as_uint(f[k]) & 0x7f800000u), dk);
@@ -1435,16 +1454,23 @@ void tend(Ctx& c, int k) {
 template <typename T, int kS>
 void launch_update_s(Ctx& c, const UpdArgs<T>& u) {
   const dim3 g(c.nblk_upd), b(kS);
+  if (u.cmminc) {  // IMMAT = 0 runs at prefetch depth 1 (set at create)
+    if (c.fuse_mom)
+      hipLaunchKernelGGL((k_update<T, 1, true, kS, true>), g, b, 0, c.stream, u);
+    else
+      hipLaunchKernelGGL((k_update<T, 1, false, kS, true>), g, b, 0, c.stream, u);
+    return;
+  }
   if (c.fuse_mom) {
     if (c.depth == 2)
-      hipLaunchKernelGGL((k_update<T, 2, true, kS>), g, b, 0, c.stream, u);
+      hipLaunchKernelGGL((k_update<T, 2, true, kS, false>), g, b, 0, c.stream, u);
     else
-      hipLaunchKernelGGL((k_update<T, 1, true, kS>), g, b, 0, c.stream, u);
+      hipLaunchKernelGGL((k_update<T, 1, true, kS, false>), g, b, 0, c.stream, u);
   } else {
     if (c.depth == 2)
-      hipLaunchKernelGGL((k_update<T, 2, false, kS>), g, b, 0, c.stream, u);
+      hipLaunchKernelGGL((k_update<T, 2, false, kS, false>), g, b, 0, c.stream, u);
     else
-      hipLaunchKernelGGL((k_update<T, 1, false, kS>), g, b, 0, c.stream, u);
+      hipLaunchKernelGGL((k_update<T, 1, false, kS, false>), g, b, 0, c.stream, u);
   }
 }
 
@@ -1453,14 +1479,19 @@ template <typename T, int kS>
 int update_blocks_per_cu_s(const Ctx& c) {
   int n = 0;
   hipError_t e;
-  if (c.fuse_mom)
+  if (c.p.prm.immat == 0)
+    e = c.fuse_mom
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 1, true, kS, true>, kS, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 1, false, kS, true>, kS,
+                                                           0);
+  else if (c.fuse_mom)
     e = c.depth == 2
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 2, true, kS>, kS, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 1, true, kS>, kS, 0);
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 2, true, kS, false>, kS, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 1, true, kS, false>, kS, 0);
   else
     e = c.depth == 2
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 2, false, kS>, kS, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 1, false, kS>, kS, 0);
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 2, false, kS, false>, kS, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_update<T, 1, false, kS, false>, kS, 0);
   return e == hipSuccess && n > 0 ? n : 1;
 }
 
@@ -1499,6 +1530,7 @@ int step_once(Ctx& c) {
   u.ephase = c.ephase;
   u.ldf = (long long)c.ldf;
   u.dither = c.dither;
+  u.cmminc = c.p.prm.immat == 0;
   tbegin(c, 0);
   launch_update<T>(c, u);
   HIPCK(hipGetLastError());
@@ -1515,6 +1547,7 @@ int step_once(Ctx& c) {
     w.alpha = (T)p.prm.alpha; w.E = p.E; w.V = p.V;
     w.ldf = (long long)c.ldf;
     w.dither = c.dither;
+    w.cmminc = c.p.prm.immat == 0;
     w.T_ = p.T; w.ngroups = c.ngroups; w.gsize = c.gsize; w.nwe = (int)p.wall_elems.size();
     w.jacobi = c.gs ? 0 : 1;
     w.diffuse = c.wall_diffuse;
@@ -1629,7 +1662,7 @@ int upload_tables(Ctx& c) {
     const double* g = &p.geome[7 * (size_t)e];
     for (int k = 0; k < 6; ++k) geo[(size_t)k * E + e] = (T)g[k];
     geo[6 * (size_t)E + e] = (T)(0.5 * g[6]);
-    geo[7 * (size_t)E + e] = (T)p.mmat[3 * (size_t)e];
+    geo[7 * (size_t)E + e] = (T)(p.prm.immat == 0 ? -p.cm12[e] : p.mmat[3 * (size_t)e]);
     for (int k = 0; k < 3; ++k) {
       const ElemEdge& ed = p.edges[3 * (size_t)e + k];
       const size_t i = (size_t)k * E + e;
@@ -1941,7 +1974,7 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   c.depth = 1;
   if (const char* pd = std::getenv("DGBGK_PREFETCH")) {
     const int d = std::atoi(pd);
-    if (d >= 1 && d <= 2) c.depth = d;
+    if (d >= 1 && d <= 2 && prm->immat != 0) c.depth = d;  // IMMAT = 0: depth 1 only
   }
   if (const char* fm = std::getenv("DGBGK_FUSE_MOMENTS")) c.fuse_mom = fm[0] != '0';
   if (const char* fd = std::getenv("DGBGK_FP32_DITHER")) c.dither = fd[0] != '0';

@@ -5,6 +5,8 @@
 //                       rows padded to ldf = 3E rounded up to 64 values, so
 //                       every row starts on a 256-byte boundary
 //   geo      T[8][E]    dN/dx (3), dN/dy (3), area, 1/lumped mass  (SoA)
+//                       (IMMAT = 0: row 7 is minus CONMAT's off-diagonal entry,
+//                       its row sum)
 //   einfo    int[3][E]  per element edge: kind | a1 | a2 | b1 | b2 | left
 //   eslot    int[3][E]  interior: neighbour's LDS slot; wall: wall-side slot
 //   enx/eny/ealen T[3][E] side normal and length/6 of each element edge
@@ -68,6 +70,7 @@ struct UpdArgs {
   int E, V, ntiles, vchunk;
   long long ldf;  // row stride of f
   int dither;     // FP32: dithered (unbiased) rounding of the f store
+  int cmminc;     // IMMAT = 0: CMMINC's increment and residual (geo row 7 its row sum)
 };
 
 template <typename T>
@@ -99,6 +102,7 @@ struct WallArgs {
   int lds_ring;  // one thread per speed ring, in-place reads staged in LDS
   long long ldf;  // row stride of f
   int dither;     // FP32: dithered (unbiased) rounding of the f store
+  int cmminc;     // IMMAT = 0: as in UpdArgs
 };
 
 template <typename T>

@@ -174,9 +174,9 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
     msg = "invalid sizes";
     return kErrArg;
   }
-  if (prm.immat != 1) {
-    msg = "only lumped mass matrices (IMMAT=1) are supported";
-    return kErrUnsupported;
+  if (prm.immat != 0 && prm.immat != 1) {
+    msg = "IMMAT must be 1 (lumped mass) or 0 (the consistent-mass branch)";
+    return kErrArg;
   }
   p = Prep();
   p.E = nelem; p.P = npoin; p.B = nboun; p.R = rorder; p.T = torder;
@@ -240,15 +240,23 @@ int prepare(int nelem, int npoin, int nboun, const int* intma, const double* coo
     }
     g[6] = rj;
   }
-  // GETLEN (src/getlen.f90:17-26), GETMAT (src/getmat.f90:8-40)
+  // GETLEN (src/getlen.f90:17-26); GETMAT (src/getmat.f90:8-40) for IMMAT =
+  // 1, else CONMAT (src/conmat.f90:20-37, src/dg_lin_convec.f90:113-117),
+  // kept as its off-diagonal entry -0.25/(RJ/24) (the diagonal is -3 times
+  // it, the same double as the reference's 0.75/(RJ/24)); MMAT stays zero
   p.relen.resize(E);
-  p.mmat.resize(3 * (size_t)E);
+  p.mmat.assign(3 * (size_t)E, 0.0);
+  if (prm.immat == 0) p.cm12.resize(E);
   for (int e = 0; e < E; ++e) {
     const double* g = &p.geome[7 * (size_t)e];
     double dist[3];
     for (int a = 0; a < 3; ++a) dist[a] = 1.0 / std::sqrt(g[a] * g[a] + g[a + 3] * g[a + 3]);
     p.relen[e] = std::min(dist[0], std::min(dist[1], dist[2]));
     double rj24 = g[6] * (1. / 24.);
+    if (prm.immat == 0) {
+      p.cm12[e] = -0.25 * (1 / rj24);
+      continue;
+    }
     for (int a = 0; a < 3; ++a) {
       double m = 0.0;
       m = m + 4.0 * rj24;

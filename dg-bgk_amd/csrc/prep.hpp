@@ -24,7 +24,7 @@ enum Status {
   kErrSumbottom = -5,      // src/flucon2.f90:227-231
   kErrReflectLookup = -6,  // src/getbou.f90:223-250 reads outside VCORD/DISNF
   kErrMesh = -7,           // ids out of range, degenerate element
-  kErrUnsupported = -8,    // IMMAT=0 (consistent mass) is not built
+  kErrUnsupported = -8,    // a configuration the library does not build
   kErrHip = -9,
   kErrNoMem = -10,
   kErrTile = -11,
@@ -58,6 +58,7 @@ struct Prep {
   // reference tables, 1-based ids, Fortran column-major
   std::vector<int> intma, bsido, iside;
   std::vector<double> coord, nx, ny, el, geome, relen, mmat, rsido, vcord;
+  std::vector<double> cm12;  // IMMAT = 0: CONMAT's off-diagonal entry per element
   double sumweight = 0.0, dtec = 0.0;
   // per ordinate: velocity in the two radius forms the reference uses
   // (A: (rv/2)*(eta+1) in ADVNCE/COMPMU, B: eta*(rv/2)+rv/2 in MACROS,

@@ -31,7 +31,7 @@ extern "C" {
 #define DGBGK_ERR_SUMBOTTOM -5      /* src/flucon2.f90:227-231 */
 #define DGBGK_ERR_REFLECT_LOOKUP -6 /* src/getbou.f90:223-250 (out of VCORD) */
 #define DGBGK_ERR_MESH -7
-#define DGBGK_ERR_UNSUPPORTED -8    /* IMMAT=0 (src/cmminc.f90) */
+#define DGBGK_ERR_UNSUPPORTED -8    /* a configuration the library does not build */
 #define DGBGK_ERR_HIP -9
 #define DGBGK_ERR_NOMEM -10
 #define DGBGK_ERR_TILE -11
@@ -53,7 +53,9 @@ typedef struct dgbgk_params {
   double t1, p1, u0, v0;               /* IVD%T1, P1 [bar], U0, V0 */
   double cinf[4];                      /* IVD%CINF: T, P [bar], U, V */
   int inf;                             /* IVD%INF */
-  int immat;                           /* IVD%IMMAT; only 1 (lumped) */
+  int immat;                           /* IVD%IMMAT: 1 lumped mass (GETINC), 0 the
+                                          consistent-mass branch (CONMAT, CMMINC,
+                                          src/advnce.f90:349-361); else an error */
   int precision;                       /* DGBGK_FP64 or DGBGK_FP32 */
   int lookup;                          /* DGBGK_LOOKUP_* */
   int order;                           /* DGBGK_ORDER_* */

@@ -65,6 +65,41 @@ def test_c1_fp64_reference_order(nsteps):
     assert rel_l2_vec(pf["uvel"], pf["vvel"], op["uvel"], op["vvel"]) < FP64_TOL
 
 
+@pytest.mark.parametrize("make", [c1, cylinder_small])
+def test_consistent_mass_branch(make):
+    """IMMAT = 0 (CONMAT + CMMINC, src/advnce.f90:349-361) against the
+    oracle: FP64 over 20 steps to 1e-10 (f, point fields, DTE, mass, and
+    SUMRES, which is CMMINC's signed largest term), the FP32-bulk form to
+    1e-5; and its increments are the lumped form's (CMMAT's row sum is the
+    lumped mass), so f is the IMMAT = 1 run's up to rounding."""
+    from dgbgk import _lib
+    from dgbgk.solver import Solver
+    case = make()
+    case.prm["immat"] = 0
+    s, o = _pair(case)
+    print(f"case {case.name} E={case.mesh.nelem} IMMAT=0 fp64 20 steps", flush=True)
+    rs, ro = s.step(20), o.step(20)
+    _compare_steps(rs, ro, FP64_TOL, MASS_TOL)
+    f = s.get_f()
+    assert rel_l2(f, o.get_f()) < FP64_TOL
+    pf, op = s.point_fields(), o.point_fields()
+    for k in ("nd", "rho", "ps", "temp"):
+        assert rel_l2(pf[k], op[k]) < FP64_TOL, k
+    o.close()
+    s.close()
+    lumped = dict(case.prm)
+    lumped["immat"] = 1
+    case1 = type(case)(case.name, case.mesh, case.eta, case.w, case.torder, lumped)
+    with Solver(case1) as s1:
+        r1 = s1.step(20)
+        assert rel_l2(s1.get_f(), f) < 1e-12
+    assert any(abs(a["sumres"] - b["sumres"]) > 1e-6 * b["sumres"] for a, b in zip(rs, r1))
+    print(f"case {case.name} IMMAT=0 fp32 20 steps", flush=True)
+    with Solver(case, precision=_lib.DGBGK_FP32) as s32:
+        s32.step(20)
+        assert rel_l2(s32.get_f(), f) < FP32_TOL
+
+
 def test_cylinder_diffuse_wall_fp64():
     case = cylinder_small()
     s, o = _pair(case)

@@ -106,10 +106,22 @@ def test_reference_ring_rule_outside_its_domain():
     assert _err(case, lookup=0) == -6
 
 
-def test_consistent_mass_is_unsupported():
+def test_mass_matrix_choice():
+    """IMMAT = 0 (the consistent-mass branch) builds; other values are
+    rejected; with IMMAT = 0 the lumped MMAT table stays zero, as in the
+    reference, which then fills CMMAT instead (src/dg_lin_convec.f90:113-117)."""
+    from dgbgk.solver import Setup
+    from oracle_py import OracleRun
     case = c1(n=4)
+    case.prm["immat"] = 2
+    assert _err(case) == -1
     case.prm["immat"] = 0
-    assert _err(case) == -8
+    t = Setup(case).tables()
+    o = OracleRun(case)
+    assert not t["mmat"].any()
+    assert np.array_equal(t["mmat"], o.tables()["mmat"])
+    assert o.cmmat().any()
+    o.close()
 
 
 def test_inflow_without_inflow_state():
