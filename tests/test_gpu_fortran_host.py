@@ -34,14 +34,18 @@ def test_fortran_host_check_mode(tmp_path):
 
 
 @pytest.mark.gpu
-def test_fortran_host_matches_oracle(tmp_path):
+@pytest.mark.parametrize("immat", [1, 0])
+def test_fortran_host_matches_oracle(tmp_path, immat):
+    """A deck with the lumped mass (IMMAT = 1) and one with the
+    consistent-mass branch (IMMAT = 0), whose residual is CMMINC's."""
     from dgbgk import io
     from oracle_py import OracleRun
     assert os.path.exists(HOST), "Fortran host not built"
     case = cylinder_small(elems=1200)
+    case.prm["immat"] = immat
     nt = 12
     deck = io.write_run_dir(case, str(tmp_path), ntime=nt, forceout=1)
-    print(f"fortran host cylinder E={case.mesh.nelem} steps={nt}", flush=True)
+    print(f"fortran host cylinder E={case.mesh.nelem} IMMAT={immat} steps={nt}", flush=True)
     out = subprocess.run([HOST, deck], cwd=tmp_path, capture_output=True, text=True,
                          timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
