@@ -227,8 +227,9 @@ def measure(case, precision: str, steps: int, warmup: int, vchunk: int = 0,
 
 
 def host_threads() -> int:
-    """Threads for the CPU baseline: the run's OMP_NUM_THREADS (the GPU box
-    sets 16, its CPU share), else the visible CPUs."""
+    """Threads for the CPU baseline: the run's OMP_NUM_THREADS (a host that
+    grants this process a share of its cores sets it), else the visible CPUs,
+    at most 16."""
     try:
         return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
     except ValueError:

@@ -450,9 +450,9 @@ def main(argv=None):
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--data", default="gpurun_out/warmstart_train.npz")
-    ap.add_argument("--params", default="gpurun_out/warmstart_params.npz")
-    ap.add_argument("--out", default="gpurun_out/warmstart_eval.json")
+    ap.add_argument("--data", default="warmstart_train.npz")
+    ap.add_argument("--params", default="warmstart_params.npz")
+    ap.add_argument("--out", default="warmstart_eval.json")
     ap.add_argument("--geometry", default="cylinder", choices=sorted(GEOMETRIES))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
                     help="generate: the form the steady states are run in")

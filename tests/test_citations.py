@@ -10,7 +10,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = os.environ.get("DGBGK_REFERENCE", "/root/reference")
-# files written by others (the survey, the reviews, the driver)
+# files this project does not write (they quote the reference freely)
 NOT_OURS = {"VERDICT.md", "ADVICE.md", "SURVEY.md", "BASELINE.md", "BASELINE.json"}
 CITE = re.compile(r"((?:src|tc/\w+|doc|tools)/[\w./-]+?\.(?:f90|txt|md|con|OUT|plt|py|gnu|inp))"
                   r"((?::\d+(?:-\d+)?)(?:,\d+(?:-\d+)?)*)")
