@@ -26,6 +26,10 @@ cpu_baseline: the FP64 C oracle, threaded over the run's OMP_NUM_THREADS
 bounded sample of the workload: the same mesh and flow with 2 x 64
 ordinates (DOF-velocity updates per second are per ordinate and element,
 so the rate carries over).
+accuracy: the oracle's result on that sample checks the library's FP64 and
+FP32-bulk forms run on the same sample (relative L2 of rho, (u, v) and T at
+the points, relative pressure drag on the current point pressure, largest
+per-step relative mass difference).
 time_to_residual (--no-ttr skips it): steps and wall time to a step change
 of 1e-6 of the plain run's step-1 change, FP64 from the freestream against
 FP32-bulk from the committed warm-start network's state, freestream held
@@ -236,12 +240,56 @@ def host_threads() -> int:
         return max(1, min(16, os.cpu_count() or 1))
 
 
+def _rel_l2(a, b) -> float:
+    import numpy as np
+    a, b = np.asarray(a, float).ravel(), np.asarray(b, float).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def sample_accuracy(case, nsteps: int, ref: dict, ref_recs: list) -> dict:
+    """Relative L2 errors of the point fields rho, (u, v) and T, the relative
+    error of the pressure drag (GETFOR on the current point pressure) and
+    the largest per-step relative mass difference of the library's FP64 and
+    FP32-bulk forms over nsteps steps of the CPU baseline's sample, against
+    the FP64 C oracle's values for the same steps (ref: its point fields
+    after the last step; ref_recs: its step records)."""
+    import ctypes as C
+    import numpy as np
+    from dgbgk import _lib
+    from dgbgk.solver import Solver
+    from oracle_py import lib as oracle_lib
+    m = case.mesh
+    lo, do = C.c_double(), C.c_double()
+    oracle_lib().ora_getfor(m.nboun, np.ascontiguousarray(m.bsido, dtype=np.int32),
+                            np.ascontiguousarray(ref["ps"], dtype=np.float64),
+                            np.ascontiguousarray(m.coord, dtype=np.float64),
+                            C.byref(lo), C.byref(do))
+    ref_drag = do.value
+    out = {"vs": f"FP64 C oracle, {nsteps} steps of the cpu_baseline sample",
+           "drag_oracle": ref_drag}
+    du = np.hypot(ref["uvel"], ref["vvel"])
+    for prec, name in ((_lib.DGBGK_FP64, "fp64"), (_lib.DGBGK_FP32, "fp32_bulk")):
+        with Solver(case, precision=prec) as s:
+            recs = s.step(nsteps)
+            pf = s.point_fields()
+            drag = s.forces()[1]
+        duv = np.hypot(pf["uvel"] - ref["uvel"], pf["vvel"] - ref["vvel"])
+        out[name] = {"rho": _rel_l2(pf["rho"], ref["rho"]),
+                     "uv": float(np.linalg.norm(duv) / max(np.linalg.norm(du), 1e-300)),
+                     "temp": _rel_l2(pf["temp"], ref["temp"]),
+                     "drag": abs(drag - ref_drag) / max(abs(ref_drag), 1e-300),
+                     "mass_per_step": max(abs(a["mass"] - b["mass"]) / abs(b["mass"])
+                                          for a, b in zip(recs, ref_recs))}
+    return out
+
+
 def cpu_baseline(config: str, elems: int, serial: bool) -> dict:
     """The FP64 C oracle on a bounded sample of the workload: five steps of
     the same mesh and flow with 2 speed rings x 64 angles (Gauss-Legendre
     radial nodes; about 10 s on 16 threads at 1M triangles), threaded over
     the host threads (bit-identical to one thread); optionally also one step
-    on one thread."""
+    on one thread.  The oracle's result then checks the library's two forms
+    on the same sample ("accuracy")."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from dgbgk import cases
     from oracle_py import OracleRun
@@ -255,8 +303,9 @@ def cpu_baseline(config: str, elems: int, serial: bool) -> dict:
     o.set_threads(nt)
     nsteps = 5
     t0 = time.perf_counter()
-    o.step(nsteps)
+    recs = o.step(nsteps)
     dt = (time.perf_counter() - t0) / nsteps
+    accuracy = sample_accuracy(case, nsteps, o.point_fields(), recs)
     cpu = None
     for line in open("/proc/cpuinfo"):
         if line.startswith("model name"):
@@ -268,7 +317,8 @@ def cpu_baseline(config: str, elems: int, serial: bool) -> dict:
                      f"oracle in the reference's order, {nt} threads, {dt * nsteps:.1f} s",
            "host": {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
                     "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
-                    "cpu": cpu or platform.processor()}}
+                    "cpu": cpu or platform.processor()},
+           "accuracy": accuracy}
     if serial:
         o.set_threads(1)
         t0 = time.perf_counter()
@@ -371,6 +421,7 @@ def main():
         "residual": main_m["residual"],
         "mass": main_m["mass"],
         "cpu_baseline": cpu,
+        "accuracy": cpu.pop("accuracy") if cpu else None,
     }
     out.update(extra)
     if not args.no_ttr and not args.only and plain and not args.elems:
