@@ -37,6 +37,8 @@ on all box sides: the particle box at the workload's velocity grid and
 flow on a 25,781-triangle mesh ("time_to_residual"), and the ~100k-triangle
 cylinder ("time_to_residual_cylinder"); about a minute each.
   python bench.py --network STEPS   times the warm start's network alone.
+  python bench.py --contexts N      the update on N contexts created one after
+                                    another (the spread between contexts).
 """
 import argparse
 import json
@@ -357,6 +359,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run three device-to-device copies of BYTES bytes (the "
                          "reference kernel that scales rocprofv3's FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--contexts", type=int, default=0, metavar="N",
+                    help="only measure the named configuration and precision on N device "
+                         "contexts created one after another (the spread between contexts)")
     ap.add_argument("--network", type=int, default=0, metavar="STEPS",
                     help="only time the warm start's network: STEPS training steps and one "
                          "inference pass (for rocprofv3 kernel traces)")
@@ -375,6 +380,13 @@ def main():
 
     case, wl, mach, kn = make_case(args.config, args.elems, args.mesh, args.shuffle)
     E, V = case.mesh.nelem, case.nvel
+    if args.contexts:
+        runs = [measure(case, args.precision, args.steps, args.warmup, args.vchunk)
+                for _ in range(args.contexts)]
+        print(json.dumps({"workload": wl, "precision": args.precision, "contexts": [
+            {"update_ms": r["roofline"]["avg_launch_ms"], "frac": r["roofline"]["frac"],
+             "ms_per_step": r["ms_per_step"]} for r in runs]}))
+        return
     plain = not args.mesh and args.shuffle < 0
     key = f"{args.config}:{E}x{V}" if plain else None
     main_m = measure(case, args.precision, args.steps, args.warmup, args.vchunk, key)
