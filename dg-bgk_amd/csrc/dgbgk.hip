@@ -746,6 +746,7 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
   const double* wv = a.vtd + 4 * V;
   const double* wcx = a.vtd + 5 * V;
   const double* wcy = a.vtd + 6 * V;
+  const double* hv = a.vtd + 7 * V;
 
   // First pass (n, n u, n v), summed per ordinate chunk of the update
   // kernel, then over the chunks: this wavefront's contiguous share of the
@@ -822,26 +823,32 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
 
   // Second pass (pressure and COMPMU's mean relative speed) over this
   // wavefront's contiguous share of the ordinates, four at a time.
+  // FP64: the reference's terms, 0.5 |c_b - u|^2 w f and |c_a - u| w f.
+  // FP32-bulk: the pressure sum expanded about the first pass's mean
+  // velocity, sum of f (0.5 w |c_b|^2) here and 0.5 n |u|^2 subtracted once
+  // (the two agree in exact arithmetic since n u = sum w c_b f), and the
+  // relative speed in single precision (the hardware square root, far
+  // below the FP32 rounding of f); both sums in FP64, compensated.
+  const float uxf = (float)ux, uyf = (float)uy;
   double sp = 0, cp = 0, st = 0, ct = 0;
+  auto term = [&](T raw, int v, double& bp, double& bt) {
+    if constexpr (kComp) {
+      const float ex = uxf - (float)ldc(a.vtab, v), ey = uyf - (float)ldc(a.vtab, V + v);
+      const float rel = __builtin_amdgcn_sqrtf(fmaf(ex, ex, ey * ey));
+      bp = fma((double)raw, ldc(hv, v), bp);
+      bt = fma((double)(rel * (float)raw), ldc(wv, v), bt);
+    } else {
+      const double fw = (double)raw * ldc(wv, v);
+      const double dx = ldc(cxb, v) - ux, dy = ldc(cyb, v) - uy;
+      const double ex = ux - ldc(cxa, v), ey = uy - ldc(cya, v);
+      sp += 0.5 * (dx * dx + dy * dy) * fw;
+      if (dens) st += sqrt(ex * ex + ey * ey) * fw;
+    }
+  };
   auto block4 = [&](const T (&raw)[4], int v) {
     double bp = 0, bt = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const double fw = (double)raw[j] * ldc(wv, v + j);
-      const double dx = ldc(cxb, v + j) - ux, dy = ldc(cyb, v + j) - uy;
-      const double ex = ux - ldc(cxa, v + j), ey = uy - ldc(cya, v + j);
-      const double e2 = ex * ex + ey * ey;
-      // FP32-bulk: the hardware square root (single-precision accuracy,
-      // far below the FP32 rounding of f); FP64: correctly rounded
-      const double rel = kComp ? __builtin_amdgcn_sqrt(e2) : sqrt(e2);
-      if (kComp) {
-        bp += 0.5 * (dx * dx + dy * dy) * fw;
-        bt += rel * fw;
-      } else {
-        sp += 0.5 * (dx * dx + dy * dy) * fw;
-        if (dens) st += rel * fw;
-      }
-    }
+    for (int j = 0; j < 4; ++j) term(raw[j], v + j, bp, bt);
     if (kComp) {
       nsum<true>(sp, cp, bp);
       if (dens) nsum<true>(st, ct, bt);
@@ -892,13 +899,11 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
       block4(rc, v);
     }
     for (; v < ve; ++v) {
-      const double fw = (double)fp[(size_t)v * a.ldf] * ldc(wv, v);
-      const double dx = ldc(cxb, v) - ux, dy = ldc(cyb, v) - uy;
-      nsum<kComp>(sp, cp, 0.5 * (dx * dx + dy * dy) * fw);
-      if (dens) {
-        const double ex = ux - ldc(cxa, v), ey = uy - ldc(cya, v);
-        const double e2 = ex * ex + ey * ey;
-        nsum<kComp>(st, ct, (kComp ? __builtin_amdgcn_sqrt(e2) : sqrt(e2)) * fw);
+      double bp = 0, bt = 0;
+      term(fp[(size_t)v * a.ldf], v, bp, bt);
+      if (kComp) {
+        nsum<true>(sp, cp, bp);
+        if (dens) nsum<true>(st, ct, bt);
       }
     }
   }
@@ -919,7 +924,7 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
   }
   double mu = 0.0, mass = 0.0;
   if (sub == 0 && live) {
-    const double ps = (tp - ep) * a.mm;
+    const double ps = ((tp - ep) - (kComp ? 0.5 * nd * (ux * ux + uy * uy) : 0.0)) * a.mm;
     const double ctv = dens ? (tt - et) / n : 0.0;
     mu = sqrt(2.0) * ctv * a.sigt * nd;
     a.mom[k] = nd;
@@ -1376,6 +1381,7 @@ int launch_moments(Ctx& c, bool partials = false) {
   MomArgs<T> m;
   m.f = (const T*)c.f[c.cur];
   m.vtd = c.vtd;
+  m.vtab = (const T*)c.vtab;
   m.rj = c.rj;
   m.mom = c.mom;
   m.np = (T*)c.np;
@@ -1708,7 +1714,7 @@ int upload_tables(Ctx& c) {
   if ((st = upload((T**)&c.uny, uny, c.owned))) return st;
   if ((st = upload(&c.uinfo, uinfo, c.owned))) return st;
   std::vector<T> vtab(4 * (size_t)V);
-  std::vector<double> vtd(7 * (size_t)V);
+  std::vector<double> vtd(8 * (size_t)V);
   for (int v = 0; v < V; ++v) {
     vtab[v] = (T)p.cxa[v];
     vtab[V + v] = (T)p.cya[v];
@@ -1724,6 +1730,8 @@ int upload_tables(Ctx& c) {
     // the first moment pass's weights: n u = sum (w cx) f, n v = sum (w cy) f
     vtd[5 * V + v] = p.wv[v] * p.cxb[v];
     vtd[6 * V + v] = p.wv[v] * p.cyb[v];
+    // the FP32-bulk pressure sum's weight: 0.5 w |c_b|^2
+    vtd[7 * V + v] = 0.5 * p.wv[v] * (p.cxb[v] * p.cxb[v] + p.cyb[v] * p.cyb[v]);
   }
   if ((st = upload((T**)&c.vtab, vtab, c.owned))) return st;
   if ((st = upload(&c.vtd, vtd, c.owned))) return st;

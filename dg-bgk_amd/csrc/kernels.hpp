@@ -61,7 +61,7 @@ struct UpdArgs {
   const int* __restrict__ tiles;   // [ntiles][4] e0, nown, hoff, nhalo
   const int* __restrict__ halo;
   T* __restrict__ wpart;           // [NWE][V][8] partial RHS (3), COLL (3), U
-  const double* __restrict__ vtd;  // [7][V]: cxa, cya, cxb, cyb, wv, wv cxb, wv cyb
+  const double* __restrict__ vtd;  // [8][V]: cxa, cya, cxb, cyb, wv, wv cxb, wv cyb, ...
   double* __restrict__ msum;       // [nvchunks][3][3E] moment partials of f^{k+1}
   const double* __restrict__ scal; // [0] = DTE
   double* __restrict__ res_part;   // per block: max|increment|; +res_stride: max|change|
@@ -108,7 +108,9 @@ struct WallArgs {
 template <typename T>
 struct MomArgs {
   const T* __restrict__ f;
-  const double* __restrict__ vtd;  // [7][V]: cxa, cya, cxb, cyb, wv, wv cxb, wv cyb
+  const double* __restrict__ vtd;  // [8][V]: cxa, cya, cxb, cyb, wv, wv cxb, wv cyb,
+                                   //        0.5 wv |c_b|^2
+  const T* __restrict__ vtab;      // [4][V]: cxa, cya as T (rows 0, 1)
   const double* __restrict__ rj;   // GEOME(7,e)
   double* __restrict__ mom;        // [5][3E]
   T* __restrict__ np;              // [5][3E]
