@@ -325,7 +325,11 @@ __device__ __forceinline__ void update_ordinate(const UpdArgs<T>& a, const SlotS
       const bool own = (in & kUpdOwn) || w < T(0) || ((in & kUpdRight) && w == T(0));
       const T n1 = sf[s.nb1[k]], n2 = sf[s.nb2[k]], un = sf[s.nb1[k] | 3];
       const T du = un - U;
-      const T e1 = ((n1 - f[0]) - db) - du, e2 = ((n2 - f[0]) - db) - du;
+      T e1 = ((n1 - f[0]) - db) - du, e2 = ((n2 - f[0]) - db) - du;
+      // every lane forms the neighbour's values (the reads are in bounds
+      // for all slots), so the upwind choice below is a select, not a
+      // divergent branch around the reads per edge
+      asm volatile("" : "+v"(e1), "+v"(e2));
       const bool inflow = (in & kUpdInflow) != 0;
       const T g1 = own ? (inflow ? -gbar : h[k]) : e1;
       const T g2 = own ? (inflow ? -gbar : h[k1]) : e2;
