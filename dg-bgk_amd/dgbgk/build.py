@@ -90,10 +90,26 @@ def build_stream_ceiling(verbose: bool = False) -> str:
     return STREAM
 
 
+ACCESS = os.path.join(PKG, "access_probe")
+
+
+def build_access_probe(verbose: bool = False) -> str:
+    """The update kernel's access structure without its arithmetic
+    (csrc/access_probe.hip), an executable."""
+    tmp = ACCESS + ".tmp"
+    _run([HIPCC] + [f for f in HIP_FLAGS if f != "-fPIC"] +
+         [os.path.join(CSRC, "access_probe.hip"), "-o", tmp])
+    os.replace(tmp, ACCESS)
+    if verbose:
+        print("built", ACCESS)
+    return ACCESS
+
+
 def build_all(verbose: bool = False):
     build_lib(verbose)
     build_fortran_host(verbose)
     build_stream_ceiling(verbose)
+    build_access_probe(verbose)
 
 
 if __name__ == "__main__":
