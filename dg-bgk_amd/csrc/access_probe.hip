@@ -6,8 +6,9 @@
 // writes them to the same place in the second buffer, as k_update reads f^k
 // and writes f^{k+1}.  Variants: doubles per lane (3, as an element's three
 // nodes, or 2), an LDS exchange and barrier per ordinate (as k_update's),
-// the next ordinate's loads in flight (as k_update's prefetch), and
-// non-temporal or plain stores.  Each variant: one warm-up launch, then the
+// the next ordinate's loads in flight (as k_update's prefetch),
+// non-temporal or plain stores, and, as a layout alternative, f stored
+// tile-major (each tile's ordinates contiguous).  Each variant: one warm-up launch, then the
 // mean of 3 launches timed with HIP events; bytes = one read + one write.
 //
 //   access_probe
@@ -28,7 +29,7 @@
 
 constexpr int kSlots = 256;
 
-template <int kW, bool kBar, bool kPre, bool kNT>
+template <int kW, bool kBar, bool kPre, bool kNT, bool kTileMajor = false>
 __global__ void __launch_bounds__(kSlots) walk(const double* __restrict__ a,
                                                double* __restrict__ b, int ntiles, int V,
                                                int vchunk) {
@@ -36,8 +37,11 @@ __global__ void __launch_bounds__(kSlots) walk(const double* __restrict__ a,
   const int v0 = c * vchunk, v1 = min(V, v0 + vchunk);
   __shared__ double sh[kSlots];
   double acc = 0;
-  const size_t row = (size_t)ntiles * kW * kSlots;
-  const size_t base = (size_t)tile * kW * kSlots + kW * threadIdx.x;
+  // ordinate stride and this lane's offset: velocity-major rows of all
+  // tiles, or each tile's ordinates contiguous
+  const size_t row = kTileMajor ? (size_t)kW * kSlots : (size_t)ntiles * kW * kSlots;
+  const size_t base = (kTileMajor ? (size_t)tile * V * kW * kSlots : (size_t)tile * kW * kSlots) +
+                      kW * threadIdx.x;
   double x[kW], y[kW];
   for (int j = 0; j < kW; ++j) x[j] = a[(size_t)v0 * row + base + j];
   for (int v = v0; v < v1; ++v) {
@@ -91,6 +95,7 @@ int main() {
       {"3/lane barrier prefetch plain", walk<3, true, true, false>, 3},
       {"2/lane barrier prefetch nt", walk<2, true, true, true>, 2},
       {"2/lane prefetch nt", walk<2, false, true, true>, 2},
+      {"3/lane barrier prefetch nt, tile-major", walk<3, true, true, true, true>, 3},
   };
   const dim3 grid(ntiles * nchunks), block(kSlots);
   std::printf("buffers: 2 x %.1f GB; %d tiles x %d slots, %d ordinates in %d chunks\n",
@@ -107,7 +112,7 @@ int main() {
       float ms = 0.f;
       CK(hipEventElapsedTime(&ms, e0, e1));
       ms /= 3.f;
-      std::printf("rep %d %-32s %8.3f ms %8.1f GB/s\n", rep, x.name, ms, bytes / ms / 1e6);
+      std::printf("rep %d %-40s %8.3f ms %8.1f GB/s\n", rep, x.name, ms, bytes / ms / 1e6);
       std::fflush(stdout);
     }
   CK(hipFree(a));
