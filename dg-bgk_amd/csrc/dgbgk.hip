@@ -827,7 +827,8 @@ __global__ void __launch_bounds__(256) k_moments(MomArgs<T> a) {
 
   // Second pass (pressure and COMPMU's mean relative speed) over this
   // wavefront's contiguous share of the ordinates, four at a time.
-  // FP64: the reference's terms, 0.5 |c_b - u|^2 w f and |c_a - u| w f.
+  // FP64: the reference's terms, 0.5 |c_b - u|^2 w f (src/macros.f90:144-148)
+  // and |c_a - u| w f (COMPMU's mean relative speed).
   // FP32-bulk: the pressure sum expanded about the first pass's mean
   // velocity, sum of f (0.5 w |c_b|^2) here and 0.5 n |u|^2 subtracted once
   // (the two agree in exact arithmetic since n u = sum w c_b f), and the
