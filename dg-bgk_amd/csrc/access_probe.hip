@@ -8,7 +8,9 @@
 // nodes, or 2), an LDS exchange and barrier per ordinate (as k_update's),
 // the next ordinate's loads in flight (as k_update's prefetch),
 // non-temporal or plain stores, and, as a layout alternative, f stored
-// tile-major (each tile's ordinates contiguous).  Each variant: one warm-up launch, then the
+// tile-major (each tile's ordinates contiguous); then the FP32-bulk shape
+// (3 floats per lane, 7 chunks of 293 ordinates) in the first half of
+// each buffer.  Each variant: one warm-up launch, then the
 // mean of 3 launches timed with HIP events; bytes = one read + one write.
 //
 //   access_probe
@@ -29,20 +31,19 @@
 
 constexpr int kSlots = 256;
 
-template <int kW, bool kBar, bool kPre, bool kNT, bool kTileMajor = false>
-__global__ void __launch_bounds__(kSlots) walk(const double* __restrict__ a,
-                                               double* __restrict__ b, int ntiles, int V,
-                                               int vchunk) {
+template <int kW, bool kBar, bool kPre, bool kNT, bool kTileMajor = false, typename T = double>
+__global__ void __launch_bounds__(kSlots) walk(const T* __restrict__ a, T* __restrict__ b,
+                                               int ntiles, int V, int vchunk) {
   const int tile = blockIdx.x % ntiles, c = blockIdx.x / ntiles;
   const int v0 = c * vchunk, v1 = min(V, v0 + vchunk);
-  __shared__ double sh[kSlots];
-  double acc = 0;
+  __shared__ T sh[kSlots];
+  T acc = 0;
   // ordinate stride and this lane's offset: velocity-major rows of all
   // tiles, or each tile's ordinates contiguous
   const size_t row = kTileMajor ? (size_t)kW * kSlots : (size_t)ntiles * kW * kSlots;
   const size_t base = (kTileMajor ? (size_t)tile * V * kW * kSlots : (size_t)tile * kW * kSlots) +
                       kW * threadIdx.x;
-  double x[kW], y[kW];
+  T x[kW], y[kW];
   for (int j = 0; j < kW; ++j) x[j] = a[(size_t)v0 * row + base + j];
   for (int v = v0; v < v1; ++v) {
     if (kPre) {  // clamped, so the load is unconditional
@@ -56,21 +57,22 @@ __global__ void __launch_bounds__(kSlots) walk(const double* __restrict__ a,
       __syncthreads();
       acc += sh[(threadIdx.x + 1) % kSlots];
     }
-    double* q = b + (size_t)v * row + base;
+    T* q = b + (size_t)v * row + base;
     for (int j = 0; j < kW; ++j) {
       if (kNT)
-        __builtin_nontemporal_store(x[j] + 1.0, q + j);
+        __builtin_nontemporal_store(x[j] + T(1), q + j);
       else
-        q[j] = x[j] + 1.0;
+        q[j] = x[j] + T(1);
     }
     if (kBar) __syncthreads();
     if (kPre)
       for (int j = 0; j < kW; ++j) x[j] = y[j];
   }
-  if (acc == -1.0) b[0] = acc;  // keeps the LDS exchange
+  if (acc == T(-1)) b[0] = acc;  // keeps the LDS exchange
 }
 
 typedef void (*Kern)(const double*, double*, int, int, int);
+typedef void (*KernF)(const float*, float*, int, int, int);
 
 int main() {
   const int ntiles = 4907, V = 2048, vchunk = 410;
@@ -115,6 +117,37 @@ int main() {
       std::printf("rep %d %-40s %8.3f ms %8.1f GB/s\n", rep, x.name, ms, bytes / ms / 1e6);
       std::fflush(stdout);
     }
+  // the FP32-bulk shape: 3 floats per lane (f of one element), the FP32
+  // update's 7 ordinate chunks of 293
+  {
+    const int vchunk32 = 293, nch32 = (V + vchunk32 - 1) / vchunk32;
+    const dim3 grid32(ntiles * nch32);
+    struct VariantF {
+      const char* name;
+      KernF f;
+    } vf[] = {
+        {"fp32 3/lane barrier prefetch nt", walk<3, true, true, true, false, float>},
+        {"fp32 3/lane barrier prefetch plain", walk<3, true, true, false, false, float>},
+    };
+    for (int rep = 0; rep < 2; ++rep)
+      for (const VariantF& x : vf) {
+        const double bytes = 2.0 * ntiles * V * 3 * kSlots * sizeof(float);
+        hipLaunchKernelGGL(x.f, grid32, block, 0, 0, (const float*)a, (float*)b, ntiles, V,
+                           vchunk32);
+        CK(hipGetLastError());
+        CK(hipEventRecord(e0));
+        for (int it = 0; it < 3; ++it)
+          hipLaunchKernelGGL(x.f, grid32, block, 0, 0, (const float*)a, (float*)b, ntiles, V,
+                             vchunk32);
+        CK(hipEventRecord(e1));
+        CK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, e0, e1));
+        ms /= 3.f;
+        std::printf("rep %d %-40s %8.3f ms %8.1f GB/s\n", rep, x.name, ms, bytes / ms / 1e6);
+        std::fflush(stdout);
+      }
+  }
   CK(hipFree(a));
   CK(hipFree(b));
   return 0;
