@@ -1453,12 +1453,16 @@ void tbegin(Ctx& c, int k) {
     c.ev_pool.push_back(e);
   }
   c.ev_pool[c.ev_used].kernel = k;
-  hipEventRecord(c.ev_pool[c.ev_used].a, c.stream);
+  if (hipEventRecord(c.ev_pool[c.ev_used].a, c.stream) != hipSuccess) c.timing = false;
 }
 void tend(Ctx& c, int k) {
   if (!c.timing) return;
   (void)k;
-  hipEventRecord(c.ev_pool[c.ev_used].b, c.stream);
+  // a failed record leaves the pair unused: timing stops, the step goes on
+  if (hipEventRecord(c.ev_pool[c.ev_used].b, c.stream) != hipSuccess) {
+    c.timing = false;
+    return;
+  }
   c.ev_used += 1;
 }
 
@@ -1594,12 +1598,12 @@ int capture_graph(Ctx& c) {
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c.stream, &g);
   if (st) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);  // the step's own error is the one reported
     return st;
   }
   if (e != hipSuccess) return fail(kErrHip, std::string("graph capture: ") + hipGetErrorString(e));
   e = hipGraphInstantiate(&c.graph_exec[slot], g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);  // the executable graph no longer needs it
   if (e != hipSuccess) {
     c.graph_exec[slot] = nullptr;
     return fail(kErrHip, std::string("graph instantiate: ") + hipGetErrorString(e));
@@ -1992,9 +1996,10 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
   if (const char* fm = std::getenv("DGBGK_FUSE_MOMENTS")) c.fuse_mom = fm[0] != '0';
   if (const char* fd = std::getenv("DGBGK_FP32_DITHER")) c.dither = fd[0] != '0';
   const Prep& p = c.p;
+  // error path of create: release what was made, report the first error
   auto cleanup = [&](int code) {
-    for (void* q : c.owned) hipFree(q);
-    if (c.stream) hipStreamDestroy(c.stream);
+    for (void* q : c.owned) (void)hipFree(q);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
     delete h;
     return code;
   };
@@ -2117,15 +2122,16 @@ int dgbgk_create(int nelem, int npoin, int nboun, const int* intma, const double
 void dgbgk_destroy(dgbgk_ctx* h) {
   if (!h) return;
   Ctx& c = h->c;
-  if (c.stream) hipStreamSynchronize(c.stream);
+  // teardown has no caller to report to: every release is attempted
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
   for (auto& e : c.ev_pool) {
-    hipEventDestroy(e.a);
-    hipEventDestroy(e.b);
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
   }
   for (auto& g : c.graph_exec)
-    if (g) hipGraphExecDestroy(g);
-  for (void* q : c.owned) hipFree(q);
-  if (c.stream) hipStreamDestroy(c.stream);
+    if (g) (void)hipGraphExecDestroy(g);
+  for (void* q : c.owned) (void)hipFree(q);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
   delete h;
 }
 
@@ -2171,8 +2177,8 @@ int dgbgk_get_f(dgbgk_ctx* h, double* f) {
                        (long long)c.ldf);
   hipError_t e1 = hipMemcpyAsync(f, d, n * sizeof(double), hipMemcpyDeviceToHost, c.stream);
   hipError_t e2 = hipStreamSynchronize(c.stream);
-  hipFree(d);
-  if (e1 != hipSuccess || e2 != hipSuccess) return fail(kErrHip, "get_f failed");
+  const hipError_t e3 = hipFree(d);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(kErrHip, "get_f failed");
   return kOk;
 }
 
@@ -2194,9 +2200,9 @@ int dgbgk_set_f(dgbgk_ctx* h, const double* f) {
                        (long long)c.ldf);
   int st = do_refresh(c);
   hipError_t e2 = hipStreamSynchronize(c.stream);
-  hipFree(d);
+  const hipError_t e3 = hipFree(d);
   if (st) return st;
-  if (e1 != hipSuccess || e2 != hipSuccess) return fail(kErrHip, "set_f failed");
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(kErrHip, "set_f failed");
   return kOk;
 }
 
@@ -2371,8 +2377,9 @@ int dgbgk_f_stats(dgbgk_ctx* h, double* fmin, double* fmax, long long* nonfinite
   hipError_t e1 = hipMemcpyAsync(part.data(), d, 3 * nb * sizeof(double), hipMemcpyDeviceToHost,
                                  c.stream);
   hipError_t e2 = hipStreamSynchronize(c.stream);
-  hipFree(d);
-  if (e1 != hipSuccess || e2 != hipSuccess) return fail(kErrHip, "f_stats failed");
+  const hipError_t e3 = hipFree(d);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+    return fail(kErrHip, "f_stats failed");
   double lo = INFINITY, hi = -INFINITY, bad = 0.0;
   for (size_t b = 0; b < nb; ++b) {
     lo = std::min(lo, part[3 * b]);

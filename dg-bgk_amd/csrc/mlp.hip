@@ -583,9 +583,10 @@ int dgbgk_mlp_create(int fin, int width, int hidden, int fout, int batch, dgbgk_
     h->boff.push_back(h->nparams);
     h->nparams += dn;
   }
+  // error path of create: release what was made, report the first error
   auto bail = [&](int code) {
-    for (void* q : h->owned) hipFree(q);
-    if (h->stream) hipStreamDestroy(h->stream);
+    for (void* q : h->owned) (void)hipFree(q);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return code;
   };
@@ -629,9 +630,10 @@ int dgbgk_mlp_create(int fin, int width, int hidden, int fout, int batch, dgbgk_
 
 void dgbgk_mlp_destroy(dgbgk_mlp* h) {
   if (!h) return;
-  if (h->stream) hipStreamSynchronize(h->stream);
-  for (void* q : h->owned) hipFree(q);
-  if (h->stream) hipStreamDestroy(h->stream);
+  // teardown has no caller to report to: every release is attempted
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void* q : h->owned) (void)hipFree(q);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
@@ -704,8 +706,15 @@ int dgbgk_mlp_loss_grad(dgbgk_mlp* h, int rows, const float* x, const float* y, 
 // training set kept on the device: x [n][fin], y [n][fout]
 int dgbgk_mlp_set_data(dgbgk_mlp* h, long n, const float* x, const float* y) {
   if (!h || !x || !y || n < 1) return mfail(DGBGK_ERR_ARG, "bad arguments");
-  if (h->Xall) { hipFree(h->Xall); h->owned.erase(std::find(h->owned.begin(), h->owned.end(), (void*)h->Xall)); }
-  if (h->Yall) { hipFree(h->Yall); h->owned.erase(std::find(h->owned.begin(), h->owned.end(), (void*)h->Yall)); }
+  // the previous training set, replaced
+  if (h->Xall) {
+    MCK(hipFree(h->Xall));
+    h->owned.erase(std::find(h->owned.begin(), h->owned.end(), (void*)h->Xall));
+  }
+  if (h->Yall) {
+    MCK(hipFree(h->Yall));
+    h->owned.erase(std::find(h->owned.begin(), h->owned.end(), (void*)h->Yall));
+  }
   h->Xall = h->Yall = nullptr;
   int st = dalloc(h, &h->Xall, (size_t)n * h->fin);
   st |= dalloc(h, &h->Yall, (size_t)n * h->fout);
@@ -742,9 +751,10 @@ int dgbgk_mlp_train(dgbgk_mlp* h, int steps, float lr, unsigned seed, float* los
     if (e != hipSuccess) st = mfail(DGBGK_ERR_HIP, "loss copy failed");
   }
   hipError_t e2 = hipStreamSynchronize(h->stream);
-  if (dl) hipFree(dl);
+  const hipError_t e3 = dl ? hipFree(dl) : hipSuccess;
   if (st) return st;
   if (e2 != hipSuccess) return mfail(DGBGK_ERR_HIP, hipGetErrorString(e2));
+  if (e3 != hipSuccess) return mfail(DGBGK_ERR_HIP, hipGetErrorString(e3));
   return 0;
 }
 
